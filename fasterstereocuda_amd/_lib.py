@@ -1,0 +1,114 @@
+"""ctypes binding of libfsgm.so (the C ABI declared in include/sgm_capi.h).
+
+The library is built in-tree by ``fasterstereocuda_amd.build``.  Loading
+fails loudly when it is missing: there is no CPU fallback anywhere in the
+product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libfsgm.so")
+
+SGM_OK = 0
+SGM_E_INVALID_ARG = -1
+SGM_E_HIP = -2
+SGM_E_NOT_INIT = -3
+SGM_E_UNSUPPORTED = -4
+SGM_E_NO_DEVICE = -5
+
+INVALID_FLOAT = -999999.0
+INVALID_SHORT = -32768
+LEVEL_RANGE = 32
+
+
+class SgmOption(ctypes.Structure):
+    """Field-for-field mirror of sgm_option (CuSGMOption)."""
+    _fields_ = [
+        ("p1", ctypes.c_int32),
+        ("p2_init", ctypes.c_int32),
+        ("p2_max", ctypes.c_int32),
+        ("cs_type", ctypes.c_int32),
+        ("num_paths", ctypes.c_int32),
+        ("do_median_filter", ctypes.c_int32),
+        ("do_lr_check", ctypes.c_int32),
+        ("lr_threshold", ctypes.c_float),
+        ("unique_threshold", ctypes.c_float),
+        ("using_constant_p2", ctypes.c_int32),
+        ("do_remove_peaks", ctypes.c_int32),
+        ("peaks_ratio_threshold", ctypes.c_float),
+        ("post_filter_type", ctypes.c_int32),
+        ("morphology_type", ctypes.c_int32),
+    ]
+
+
+class SgmRoi(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int32), ("y", ctypes.c_int32),
+                ("w", ctypes.c_int32), ("h", ctypes.c_int32)]
+
+
+class SgmError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"SGM error {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int32
+F32 = ctypes.c_float
+PP4 = P * 4
+
+# name -> (restype, argtypes); the list is the C ABI of include/sgm_capi.h
+SIGNATURES = {
+    "sgm_create": (P, []),
+    "sgm_destroy": (None, [P]),
+    "sgm_init": (ctypes.c_int, [P, I32, I32, I32, I32, ctypes.POINTER(SgmOption), I32]),
+    "sgm_match": (ctypes.c_int, [P, P, P, P, P, P, P, P]),
+    "sgm_match_device": (ctypes.c_int, [P, P, P, P, P, P]),
+    "sgm_set_min_disparity": (ctypes.c_int, [P, I32]),
+    "sgm_get_disparity_range": (I32, [P]),
+    "sgm_release": (ctypes.c_int, [P]),
+    "sgm_last_error": (ctypes.c_char_p, [P]),
+    "sgm_host_alloc": (ctypes.c_int, [ctypes.POINTER(P), ctypes.c_size_t]),
+    "sgm_host_free": (ctypes.c_int, [P]),
+    "sgm_roi_from_disparity": (None, [P, I32, I32, ctypes.POINTER(SgmRoi)]),
+    "sgm_invalid_float": (F32, []),
+    "sgm_invalid_short": (ctypes.c_int16, []),
+    "sgm_level_range": (ctypes.c_int16, []),
+    "sgm_stage_census": (ctypes.c_int, [P, I32, I32, I32, I32, P, P]),
+    "sgm_stage_cost": (ctypes.c_int, [P, P, I32, I32, I32, I32, I32, I32, P, P]),
+    "sgm_stage_aggregate": (ctypes.c_int, [P, I32, I32, I32, I32, I32, I32, PP4, P]),
+    "sgm_stage_wta": (ctypes.c_int, [PP4, I32, I32, I32, I32, F32, P, P]),
+    "sgm_stage_post": (ctypes.c_int, [P, P, I32, I32, I32, F32, I32, P, P, P]),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Load libfsgm.so.  Raises ImportError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m fasterstereocuda_amd.build` "
+                "(the SGM engine has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error(handle=None) -> str:
+    msg = lib().sgm_last_error(handle)
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, handle=None) -> None:
+    if rc != SGM_OK:
+        raise SgmError(rc, last_error(handle))

@@ -1,0 +1,529 @@
+// C ABI of the SGM stereo engine: handle lifecycle, device memory, plans,
+// the StereoCuda-style Init / Match pipeline and the stage entry points.
+//
+// Pipeline per Match (ethan-li-coding/FasterStereoCuda,
+// SgmStereoCuda/StereoCudaImpl.cpp:507-546): census of both views, cost of
+// both views, 4-direction aggregation of both views, WTA of both views, LR
+// check + median.  The reference runs the right view after the left one on
+// a shared cost buffer; here both views are jobs of the same launches.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sgm_capi.h"
+#include "sgm_internal.h"
+
+using namespace fsgm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(std::string* dst, int code, const std::string& msg) {
+    if (dst) *dst = msg;
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(dst, expr)                                                            \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail((dst), SGM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+bool valid_range(int D) { return D == 64 || D == 128 || D == 256 || D == 512; }
+
+// Validates the option fields the engine reads; returns SGM_OK or an error.
+int check_option(const sgm_option* o, std::string* err) {
+    if (!o) return fail(err, SGM_E_INVALID_ARG, "option pointer is NULL");
+    if (o->cs_type != 0 && o->cs_type != 1)
+        return fail(err, SGM_E_INVALID_ARG, "cs_type must be 0 (5x5) or 1 (9x7)");
+    if (o->p1 < 0 || o->p2_init < 0 || (int64_t)o->p1 + o->p2_init > 65000)
+        return fail(err, SGM_E_INVALID_ARG, "P1 and P2 must be >= 0 with P1 + P2 <= 65000");
+    if (o->do_remove_peaks)
+        return fail(err, SGM_E_UNSUPPORTED, "do_remove_peaks is not built");
+    if (o->post_filter_type != 0)
+        return fail(err, SGM_E_UNSUPPORTED, "post filters (Gauss/bilateral) are not built");
+    if (o->morphology_type != 0)
+        return fail(err, SGM_E_UNSUPPORTED, "morphology filters are not built");
+    if (!o->using_constant_p2)
+        return fail(err, SGM_E_UNSUPPORTED, "intensity-adaptive P2 is not built");
+    return SGM_OK;
+}
+
+// Uploads jobs / posts and the per-stage item tables into one device block.
+int upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost>& posts,
+                StagePlan* plan, void** block, std::string* err) {
+    const int nj = (int)jobs.size(), np = (int)posts.size();
+    std::vector<int> oc(nj + 1, 0), ok(nj + 1, 0), oa(nj + 1, 0), ow(nj + 1, 0), op(np + 1, 0);
+    int dmax = 64;
+    for (int i = 0; i < nj; ++i) {
+        oc[i + 1] = oc[i] + census_items(jobs[i]);
+        ok[i + 1] = ok[i] + cost_items(jobs[i]);
+        oa[i + 1] = oa[i] + agg_items(jobs[i]);
+        ow[i + 1] = ow[i] + wta_items(jobs[i]);
+        dmax = std::max(dmax, jobs[i].D);
+    }
+    for (int i = 0; i < np; ++i) op[i + 1] = op[i] + post_items(posts[i]);
+    const size_t s_jobs = align256(sizeof(SgmJob) * std::max(nj, 1));
+    const size_t s_posts = align256(sizeof(SgmPost) * std::max(np, 1));
+    const size_t s_oj = align256(sizeof(int) * (nj + 1));
+    const size_t s_op = align256(sizeof(int) * (np + 1));
+    const size_t total = s_jobs + s_posts + 4 * s_oj + s_op;
+    std::vector<uint8_t> host(total, 0);
+    size_t off = 0;
+    auto put = [&](const void* src, size_t n, size_t slot) {
+        std::memcpy(host.data() + off, src, n);
+        const size_t at = off;
+        off += slot;
+        return at;
+    };
+    const size_t a_jobs = put(jobs.data(), sizeof(SgmJob) * nj, s_jobs);
+    const size_t a_posts = put(posts.data(), sizeof(SgmPost) * np, s_posts);
+    const size_t a_oc = put(oc.data(), sizeof(int) * (nj + 1), s_oj);
+    const size_t a_ok = put(ok.data(), sizeof(int) * (nj + 1), s_oj);
+    const size_t a_oa = put(oa.data(), sizeof(int) * (nj + 1), s_oj);
+    const size_t a_ow = put(ow.data(), sizeof(int) * (nj + 1), s_oj);
+    const size_t a_op = put(op.data(), sizeof(int) * (np + 1), s_op);
+    if (*block) {
+        (void)hipFree(*block);
+        *block = nullptr;
+    }
+    HIP_TRY(err, hipMalloc(block, total));
+    HIP_TRY(err, hipMemcpy(*block, host.data(), total, hipMemcpyHostToDevice));
+    uint8_t* b = static_cast<uint8_t*>(*block);
+    plan->jobs = reinterpret_cast<SgmJob*>(b + a_jobs);
+    plan->posts = reinterpret_cast<SgmPost*>(b + a_posts);
+    plan->offs_census = reinterpret_cast<int*>(b + a_oc);
+    plan->offs_cost = reinterpret_cast<int*>(b + a_ok);
+    plan->offs_agg = reinterpret_cast<int*>(b + a_oa);
+    plan->offs_wta = reinterpret_cast<int*>(b + a_ow);
+    plan->offs_post = reinterpret_cast<int*>(b + a_op);
+    plan->njobs = nj;
+    plan->nposts = np;
+    plan->n_census = oc[nj];
+    plan->n_cost = ok[nj];
+    plan->n_agg = oa[nj];
+    plan->n_wta = ow[nj];
+    plan->n_post = op[np];
+    plan->dmax = dmax;
+    return SGM_OK;
+}
+
+}  // namespace
+
+// Per-view device buffers of one pair.
+struct ViewBuffers {
+    uint8_t* img = nullptr;
+    uint64_t* census = nullptr;
+    uint8_t* cost = nullptr;
+    uint8_t* path[4] = {nullptr, nullptr, nullptr, nullptr};
+    float* raw = nullptr;
+    float* out = nullptr;
+};
+
+struct sgm_handle {
+    bool inited = false;
+    int device = -1;
+    int32_t W = 0, H = 0, min_d = 0, D = 0;
+    sgm_option opt{};
+    bool print_log = false;
+    hipStream_t stream = nullptr;
+    void* arena = nullptr;
+    void* plan_block = nullptr;
+    ViewBuffers view[2];
+    StagePlan plan;
+    std::string err;
+};
+
+namespace {
+
+size_t arena_bytes(int W, int H, int D) {
+    const size_t npix = (size_t)W * H, ncell = npix * D;
+    return 2 * (align256(npix) + align256(npix * 8) + 5 * align256(ncell) + 2 * align256(npix * 4));
+}
+
+void carve(sgm_handle* h) {
+    const size_t npix = (size_t)h->W * h->H, ncell = npix * h->D;
+    uint8_t* p = static_cast<uint8_t*>(h->arena);
+    auto take = [&](size_t n) { uint8_t* r = p; p += align256(n); return r; };
+    for (int v = 0; v < 2; ++v) {
+        ViewBuffers& b = h->view[v];
+        b.img = take(npix);
+        b.census = reinterpret_cast<uint64_t*>(take(npix * 8));
+        b.cost = take(ncell);
+        for (int r = 0; r < 4; ++r) b.path[r] = take(ncell);
+        b.raw = reinterpret_cast<float*>(take(npix * 4));
+        b.out = reinterpret_cast<float*>(take(npix * 4));
+    }
+}
+
+SgmJob make_job(const sgm_handle* h, int v) {
+    const ViewBuffers& b = h->view[v];
+    SgmJob j{};
+    j.img = b.img;
+    j.census = b.census;
+    j.census_other = h->view[1 - v].census;
+    j.cost = b.cost;
+    for (int r = 0; r < 4; ++r) j.path[r] = b.path[r];
+    j.disp = b.raw;
+    j.W = h->W; j.H = h->H; j.D = h->D; j.min_d = h->min_d;
+    j.roi_x = std::max(0, h->min_d);
+    j.is_right = v;
+    j.cs_type = h->opt.cs_type;
+    j.p1 = h->opt.p1; j.p2 = h->opt.p2_init;
+    j.uniqueness = h->opt.unique_threshold;
+    return j;
+}
+
+int build_handle_plan(sgm_handle* h) {
+    std::vector<SgmJob> jobs;
+    jobs.push_back(make_job(h, 0));
+    jobs.push_back(make_job(h, 1));  // the right census is always needed
+    SgmPost p{};
+    p.raw_l = h->view[0].raw;
+    p.raw_r = h->view[1].raw;
+    p.out_l = h->view[0].out;
+    p.out_r = h->view[1].out;
+    p.W = h->W; p.H = h->H;
+    p.do_lr = h->opt.do_lr_check ? 1 : 0;
+    p.do_median = h->opt.do_median_filter ? 1 : 0;
+    p.lr_thr = h->opt.lr_threshold;
+    std::vector<SgmPost> posts{p};
+    return upload_plan(jobs, posts, &h->plan, &h->plan_block, &h->err);
+}
+
+// Runs the pipeline on the handle's own buffers.  With the LR check off the
+// right view is still censused (the left cost needs it) but not aggregated.
+int run_pipeline(sgm_handle* h, bool want_right, hipStream_t s) {
+    StagePlan full = h->plan;
+    StagePlan left_only = h->plan;
+    left_only.njobs = 1;
+    left_only.n_cost = cost_items(make_job(h, 0));
+    left_only.n_agg = agg_items(make_job(h, 0));
+    left_only.n_wta = wta_items(make_job(h, 0));
+    const StagePlan& ctw = h->opt.do_lr_check ? full : left_only;
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    auto mark = [&](const char* what) {
+        if (!h->print_log) return;
+        (void)hipStreamSynchronize(s);
+        const auto t1 = clk::now();
+        std::printf("** %-24s %.3f ms\n", what,
+                    std::chrono::duration<double, std::milli>(t1 - t0).count());
+        t0 = t1;
+    };
+    HIP_TRY(&h->err, launch_census(full, s));
+    mark("Census:");
+    HIP_TRY(&h->err, launch_cost(ctw, s));
+    mark("Cost:");
+    HIP_TRY(&h->err, launch_aggregate(ctw, s));
+    mark("Aggregate:");
+    HIP_TRY(&h->err, launch_wta(ctw, s));
+    mark("WTA/SubPixel:");
+    HIP_TRY(&h->err, launch_post(full, s));
+    if (want_right) {
+        const size_t mapbytes = (size_t)h->W * h->H * sizeof(float);
+        if (h->opt.do_lr_check) {
+            HIP_TRY(&h->err, hipMemcpyAsync(h->view[1].out, h->view[1].raw, mapbytes,
+                                            hipMemcpyDeviceToDevice, s));
+            HIP_TRY(&h->err, launch_right_invalidate(full, s));
+        } else {
+            HIP_TRY(&h->err, launch_fill(h->view[1].out, (size_t)h->W * h->H, kInvalidDisp, s));
+        }
+    }
+    mark("LRCheck/Filter:");
+    return SGM_OK;
+}
+
+int check_ready(sgm_handle* h) {
+    if (!h) return fail(nullptr, SGM_E_INVALID_ARG, "handle is NULL");
+    if (!h->inited) return fail(&h->err, SGM_E_NOT_INIT, "handle is not initialised");
+    HIP_TRY(&h->err, hipSetDevice(h->device));
+    return SGM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sgm_handle* sgm_create(void) { return new (std::nothrow) sgm_handle(); }
+
+void sgm_destroy(sgm_handle* h) {
+    if (!h) return;
+    sgm_release(h);
+    delete h;
+}
+
+int sgm_init(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity,
+             int32_t disp_range, const sgm_option* opt, int32_t print_log) {
+    if (!h) return fail(nullptr, SGM_E_INVALID_ARG, "handle is NULL");
+    // validate everything before touching the device
+    if (width <= 0 || height <= 0)
+        return fail(&h->err, SGM_E_INVALID_ARG, "image size must be positive");
+    if (!valid_range(disp_range))
+        return fail(&h->err, SGM_E_INVALID_ARG, "disp_range must be 64, 128, 256 or 512");
+    if (std::max(0, min_disparity) >= width)
+        return fail(&h->err, SGM_E_INVALID_ARG, "min_disparity leaves no column to match");
+    if (int rc = check_option(opt, &h->err)) return rc;
+    sgm_release(h);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(&h->err, SGM_E_NO_DEVICE, "no HIP device");
+    HIP_TRY(&h->err, hipGetDevice(&h->device));
+    h->W = width; h->H = height; h->min_d = min_disparity; h->D = disp_range;
+    h->opt = *opt;
+    h->print_log = print_log != 0;
+    HIP_TRY(&h->err, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIP_TRY(&h->err, hipMalloc(&h->arena, arena_bytes(width, height, disp_range)));
+    carve(h);
+    if (int rc = build_handle_plan(h)) return rc;
+    h->inited = true;
+    if (h->print_log)
+        std::printf("Init Completed., Disp: %d~%d\n", min_disparity, min_disparity + disp_range);
+    return SGM_OK;
+}
+
+int sgm_match_device(sgm_handle* h, const uint8_t* d_left, const uint8_t* d_right,
+                     float* d_disp_left, float* d_disp_right, void* stream) {
+    if (int rc = check_ready(h)) return rc;
+    if (!d_left || !d_right || !d_disp_left)
+        return fail(&h->err, SGM_E_INVALID_ARG, "image or output pointer is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t npix = (size_t)h->W * h->H;
+    HIP_TRY(&h->err, hipMemcpyAsync(h->view[0].img, d_left, npix, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(&h->err, hipMemcpyAsync(h->view[1].img, d_right, npix, hipMemcpyDeviceToDevice, s));
+    if (int rc = run_pipeline(h, d_disp_right != nullptr, s)) return rc;
+    HIP_TRY(&h->err, hipMemcpyAsync(d_disp_left, h->view[0].out, npix * 4,
+                                    hipMemcpyDeviceToDevice, s));
+    if (d_disp_right)
+        HIP_TRY(&h->err, hipMemcpyAsync(d_disp_right, h->view[1].out, npix * 4,
+                                        hipMemcpyDeviceToDevice, s));
+    return SGM_OK;
+}
+
+int sgm_match(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
+              float* disp_left, float* disp_right, const int16_t* init_disp_left,
+              const sgm_roi* roi_left, const sgm_roi* roi_right) {
+    if (int rc = check_ready(h)) return rc;
+    if (!img_left || !img_right || !disp_left)
+        return fail(&h->err, SGM_E_INVALID_ARG, "image or output pointer is NULL");
+    if (init_disp_left)
+        return fail(&h->err, SGM_E_UNSUPPORTED,
+                    "initial disparities are used through the pyramid entry points");
+    if (roi_left || roi_right) return fail(&h->err, SGM_E_UNSUPPORTED, "ROIs are not built");
+    const size_t npix = (size_t)h->W * h->H;
+    hipStream_t s = h->stream;
+    HIP_TRY(&h->err, hipMemcpyAsync(h->view[0].img, img_left, npix, hipMemcpyHostToDevice, s));
+    HIP_TRY(&h->err, hipMemcpyAsync(h->view[1].img, img_right, npix, hipMemcpyHostToDevice, s));
+    if (int rc = run_pipeline(h, disp_right != nullptr, s)) return rc;
+    HIP_TRY(&h->err, hipMemcpyAsync(disp_left, h->view[0].out, npix * 4, hipMemcpyDeviceToHost, s));
+    if (disp_right)
+        HIP_TRY(&h->err, hipMemcpyAsync(disp_right, h->view[1].out, npix * 4,
+                                        hipMemcpyDeviceToHost, s));
+    HIP_TRY(&h->err, hipStreamSynchronize(s));
+    return SGM_OK;
+}
+
+int sgm_set_min_disparity(sgm_handle* h, int32_t min_disparity) {
+    if (int rc = check_ready(h)) return rc;
+    if (std::max(0, min_disparity) >= h->W)
+        return fail(&h->err, SGM_E_INVALID_ARG, "min_disparity leaves no column to match");
+    h->min_d = min_disparity;
+    return build_handle_plan(h);
+}
+
+int32_t sgm_get_disparity_range(const sgm_handle* h) { return h ? h->D : 0; }
+
+int sgm_release(sgm_handle* h) {
+    if (!h) return fail(nullptr, SGM_E_INVALID_ARG, "handle is NULL");
+    if (h->device >= 0) (void)hipSetDevice(h->device);
+    if (h->stream) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipStreamDestroy(h->stream);
+        h->stream = nullptr;
+    }
+    if (h->arena) { (void)hipFree(h->arena); h->arena = nullptr; }
+    if (h->plan_block) { (void)hipFree(h->plan_block); h->plan_block = nullptr; }
+    for (auto& v : h->view) v = ViewBuffers{};
+    h->plan = StagePlan{};
+    h->inited = false;
+    return SGM_OK;
+}
+
+const char* sgm_last_error(const sgm_handle* h) {
+    if (h) return h->err.c_str();
+    return g_last_error.c_str();
+}
+
+int sgm_host_alloc(void** ptr, size_t size) {
+    if (!ptr) return fail(nullptr, SGM_E_INVALID_ARG, "ptr is NULL");
+    HIP_TRY(nullptr, hipHostMalloc(ptr, size, hipHostMallocDefault));
+    return SGM_OK;
+}
+
+int sgm_host_free(void* ptr) {
+    if (!ptr) return SGM_OK;
+    HIP_TRY(nullptr, hipHostFree(ptr));
+    return SGM_OK;
+}
+
+// StereoCudaImpl::GetRoiFromDispMap (StereoCudaImpl.cpp:585-628): first and
+// last valid row by a raster scan, first and last valid column by a column
+// scan.
+void sgm_roi_from_disparity(const float* disp, int32_t width, int32_t height, sgm_roi* roi) {
+    if (!disp || !roi) return;
+    const int64_t size = (int64_t)width * height;
+    int64_t n = 0;
+    for (n = 0; n < size; ++n)
+        if (disp[n] != SGM_INVALID_FLOAT) break;
+    roi->y = (int32_t)(n / width);
+    const int64_t size0 = (int64_t)roi->y * width;
+    for (n = size - 1; n > size0; --n)
+        if (disp[n] != SGM_INVALID_FLOAT) break;
+    roi->h = (int32_t)(n / width) - roi->y + 1;
+    bool done = false;
+    for (int32_t j = 0; j < width && !done; ++j)
+        for (int32_t i = 0; i < height; ++i)
+            if (disp[(int64_t)i * width + j] != SGM_INVALID_FLOAT) { roi->x = j; done = true; break; }
+    done = false;
+    for (int32_t j = width - 1; j > roi->x && !done; --j)
+        for (int32_t i = 0; i < height; ++i)
+            if (disp[(int64_t)i * width + j] != SGM_INVALID_FLOAT) {
+                roi->w = j - roi->x + 1; done = true; break;
+            }
+}
+
+float sgm_invalid_float(void) { return SGM_INVALID_FLOAT; }
+int16_t sgm_invalid_short(void) { return SGM_INVALID_SHORT; }
+int16_t sgm_level_range(void) { return SGM_LEVEL_RANGE; }
+
+// ------------------------------------------------------------ stage entries
+namespace {
+
+struct TmpPlan {
+    void* block = nullptr;
+    StagePlan plan;
+    ~TmpPlan() { if (block) (void)hipFree(block); }
+};
+
+SgmJob stage_job(int32_t W, int32_t H, int32_t min_d, int32_t D) {
+    SgmJob j{};
+    j.W = W; j.H = H; j.D = D; j.min_d = min_d;
+    j.roi_x = std::max(0, min_d);
+    return j;
+}
+
+int stage_check(int32_t W, int32_t H, int32_t min_d, int32_t D) {
+    if (W <= 0 || H <= 0) return fail(nullptr, SGM_E_INVALID_ARG, "image size must be positive");
+    if (!valid_range(D)) return fail(nullptr, SGM_E_INVALID_ARG, "disp_range must be 64..512, 2^n");
+    if (std::max(0, min_d) >= W) return fail(nullptr, SGM_E_INVALID_ARG, "bad min_disparity");
+    return SGM_OK;
+}
+
+int run_stage(const SgmJob& j, const SgmPost* post, hipError_t (*fn)(const StagePlan&, hipStream_t),
+              void* stream) {
+    TmpPlan t;
+    std::vector<SgmJob> jobs{j};
+    std::vector<SgmPost> posts;
+    if (post) posts.push_back(*post);
+    if (int rc = upload_plan(jobs, posts, &t.plan, &t.block, nullptr)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(nullptr, fn(t.plan, s));
+    HIP_TRY(nullptr, hipStreamSynchronize(s));
+    return SGM_OK;
+}
+
+}  // namespace
+
+int sgm_stage_census(const uint8_t* d_img, int32_t width, int32_t height, int32_t cs_type,
+                     int32_t is_right, uint64_t* d_census, void* stream) {
+    if (int rc = stage_check(width, height, 0, 64)) return rc;
+    if (!d_img || !d_census) return fail(nullptr, SGM_E_INVALID_ARG, "NULL pointer");
+    SgmJob j = stage_job(width, height, 0, 64);
+    j.img = d_img; j.census = d_census; j.cs_type = cs_type ? 1 : 0; j.is_right = is_right ? 1 : 0;
+    return run_stage(j, nullptr, launch_census, stream);
+}
+
+int sgm_stage_cost(const uint64_t* d_census_base, const uint64_t* d_census_match,
+                   int32_t width, int32_t height, int32_t min_disparity, int32_t disp_range,
+                   int32_t left_to_right, int32_t cs_type, uint8_t* d_cost, void* stream) {
+    if (int rc = stage_check(width, height, min_disparity, disp_range)) return rc;
+    if (!d_census_base || !d_census_match || !d_cost)
+        return fail(nullptr, SGM_E_INVALID_ARG, "NULL pointer");
+    SgmJob j = stage_job(width, height, min_disparity, disp_range);
+    j.census = const_cast<uint64_t*>(d_census_base);
+    j.census_other = d_census_match;
+    j.cost = d_cost;
+    j.is_right = left_to_right ? 0 : 1;
+    j.cs_type = cs_type ? 1 : 0;
+    return run_stage(j, nullptr, launch_cost, stream);
+}
+
+int sgm_stage_aggregate(const uint8_t* d_cost, int32_t width, int32_t height,
+                        int32_t min_disparity, int32_t disp_range, int32_t p1, int32_t p2,
+                        uint8_t* const d_paths[4], void* stream) {
+    if (int rc = stage_check(width, height, min_disparity, disp_range)) return rc;
+    if (!d_cost || !d_paths) return fail(nullptr, SGM_E_INVALID_ARG, "NULL pointer");
+    if (p1 < 0 || p2 < 0 || (int64_t)p1 + p2 > 65000)
+        return fail(nullptr, SGM_E_INVALID_ARG, "P1 and P2 must be >= 0 with P1 + P2 <= 65000");
+    SgmJob j = stage_job(width, height, min_disparity, disp_range);
+    j.cost = const_cast<uint8_t*>(d_cost);
+    for (int r = 0; r < 4; ++r) {
+        if (!d_paths[r]) return fail(nullptr, SGM_E_INVALID_ARG, "NULL path pointer");
+        j.path[r] = d_paths[r];
+    }
+    j.p1 = p1; j.p2 = p2;
+    return run_stage(j, nullptr, launch_aggregate, stream);
+}
+
+int sgm_stage_wta(uint8_t* const d_paths[4], int32_t width, int32_t height,
+                  int32_t min_disparity, int32_t disp_range, float uniqueness,
+                  float* d_disp, void* stream) {
+    if (int rc = stage_check(width, height, min_disparity, disp_range)) return rc;
+    if (!d_paths || !d_disp) return fail(nullptr, SGM_E_INVALID_ARG, "NULL pointer");
+    SgmJob j = stage_job(width, height, min_disparity, disp_range);
+    for (int r = 0; r < 4; ++r) {
+        if (!d_paths[r]) return fail(nullptr, SGM_E_INVALID_ARG, "NULL path pointer");
+        j.path[r] = d_paths[r];
+    }
+    j.disp = d_disp;
+    j.uniqueness = uniqueness;
+    return run_stage(j, nullptr, launch_wta, stream);
+}
+
+int sgm_stage_post(const float* d_raw_left, const float* d_raw_right, int32_t width,
+                   int32_t height, int32_t do_lr_check, float lr_threshold,
+                   int32_t do_median, float* d_out_left, float* d_out_right, void* stream) {
+    if (width <= 0 || height <= 0) return fail(nullptr, SGM_E_INVALID_ARG, "bad size");
+    if (!d_raw_left || !d_out_left || (do_lr_check && !d_raw_right))
+        return fail(nullptr, SGM_E_INVALID_ARG, "NULL pointer");
+    SgmPost p{};
+    p.raw_l = d_raw_left; p.raw_r = d_raw_right; p.out_l = d_out_left; p.out_r = d_out_right;
+    p.W = width; p.H = height; p.do_lr = do_lr_check ? 1 : 0; p.do_median = do_median ? 1 : 0;
+    p.lr_thr = lr_threshold;
+    SgmJob dummy = stage_job(width, height, 0, 64);
+    TmpPlan t;
+    std::vector<SgmJob> jobs{dummy};
+    std::vector<SgmPost> posts{p};
+    if (int rc = upload_plan(jobs, posts, &t.plan, &t.block, nullptr)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(nullptr, launch_post(t.plan, s));
+    if (d_out_right) {
+        if (do_lr_check) {
+            HIP_TRY(nullptr, hipMemcpyAsync(d_out_right, d_raw_right, (size_t)width * height * 4,
+                                            hipMemcpyDeviceToDevice, s));
+            HIP_TRY(nullptr, launch_right_invalidate(t.plan, s));
+        }
+    }
+    HIP_TRY(nullptr, hipStreamSynchronize(s));
+    return SGM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Internal types shared by the kernels and the C-ABI layer.
+//
+// Work is described as a list of "jobs": one job is one view of one stereo
+// pair (the left view matched against the right, or the right view against
+// the left).  Every stage kernel walks a flat item space; per stage the host
+// builds a prefix-sum table offs[0..njobs] of items per job, and a workgroup
+// (or wave) finds its job by binary search.  The same kernels therefore run
+// one pair (the StereoCuda::Match path), a uniform batch, or a batch of
+// pairs with different W, H and D.
+#pragma once
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+namespace fsgm {
+
+constexpr float kInvalidDisp = -999999.0f;
+constexpr int kInvalidCost = 127;
+constexpr int kMaxD = 512;
+
+enum Dir { kUD = 0, kLR = 1, kDU = 2, kRL = 3 };
+
+struct SgmJob {
+    const uint8_t* img;            // this view, u8 [H][W]
+    uint64_t* census;              // this view's census, u64 [H][W]
+    const uint64_t* census_other;  // the other view's census
+    uint8_t* cost;                 // u8 [H][W][D]
+    uint8_t* path[4];              // u8 [H][W][D] per direction
+    float* disp;                   // WTA output, f32 [H][W]
+    int32_t W, H, D, min_d;
+    int32_t roi_x;                 // max(0, min_d)
+    int32_t is_right;              // 0: left view (matches x - d), 1: right view (x + d)
+    int32_t cs_type;               // 0 = 5x5, 1 = 9x7
+    int32_t p1, p2;
+    float uniqueness;
+};
+
+struct SgmPost {                   // one stereo pair: LR check + median
+    const float* raw_l;
+    const float* raw_r;            // may be null when the LR check is off
+    float* out_l;
+    float* out_r;                  // may be null
+    int32_t W, H;
+    int32_t do_lr, do_median;
+    float lr_thr;
+};
+
+// Items per job for each stage.
+inline int census_items(const SgmJob& j) { return ((j.W + 63) / 64) * ((j.H + 15) / 16); }
+inline int cost_items(const SgmJob& j) { return j.H * ((j.W - j.roi_x + 63) / 64); }
+inline int agg_items(const SgmJob& j) { return 2 * ((j.H + 3) / 4) + 2 * ((j.W - j.roi_x + 3) / 4); }
+inline int wta_items(const SgmJob& j) { return j.H * ((j.W + 15) / 16); }
+inline int post_items(const SgmPost& p) { return ((p.W + 15) / 16) * ((p.H + 15) / 16); }
+
+// Device-side launch plan: jobs, per-stage prefix sums and totals.
+struct StagePlan {
+    SgmJob* jobs = nullptr;       // device
+    int* offs_census = nullptr;   // device, njobs + 1 each
+    int* offs_cost = nullptr;
+    int* offs_agg = nullptr;
+    int* offs_wta = nullptr;
+    SgmPost* posts = nullptr;     // device
+    int* offs_post = nullptr;
+    int njobs = 0, nposts = 0;
+    int n_census = 0, n_cost = 0, n_agg = 0, n_wta = 0, n_post = 0;
+    int dmax = 64;
+};
+
+// Launchers (sgm_kernels.hip).  All enqueue on `stream`.
+hipError_t launch_census(const StagePlan& p, hipStream_t s);
+hipError_t launch_cost(const StagePlan& p, hipStream_t s);
+hipError_t launch_aggregate(const StagePlan& p, hipStream_t s);
+hipError_t launch_wta(const StagePlan& p, hipStream_t s);
+hipError_t launch_post(const StagePlan& p, hipStream_t s);
+hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s);
+hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t s);
+
+}  // namespace fsgm

@@ -1,0 +1,585 @@
+// SGM stereo stage kernels for CDNA4 (gfx950).
+//
+// Stages (one launch each, covering every job of a plan):
+//   census      LDS-staged image tile -> 64-bit census per pixel
+//   cost        LDS-staged census rows -> u8 Hamming cost volume [H][W][D]
+//   aggregate   4-direction SGM recurrence, one wave = 4 scanlines
+//   wta         4-path sum, argmin, uniqueness, parabola subpixel
+//   post        LR consistency check + 3x3 median
+//
+// Aggregation layout: a wave64 is four 16-lane DPP rows; each row owns one
+// scanline and each lane owns D/16 consecutive disparities, held as packed
+// u16 pairs (v_pk_add_u16 / v_pk_sub_u16 / v_pk_min_u16).  The d-1 / d+1
+// neighbours crossing a lane boundary come from row_shr:1 / row_shl:1 DPP
+// moves whose "old" operand supplies the reference's INVALID_COST=127
+// boundary term; the per-step minimum is a 4-step row_ror DPP reduction,
+// so every lane of the row holds it without LDS or barriers.
+//
+// Arithmetic matches the reference exactly (ethan-li-coding/FasterStereoCuda,
+// SgmStereoCuda/cusgm_cc.cu, cusgm_ca.cu, cusgm_df.cu); see each kernel.
+#include <algorithm>
+
+#include "sgm_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace fsgm {
+namespace {
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 as_v(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
+__device__ __forceinline__ uint32_t as_u(u16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) { return as_u(as_v(a) + as_v(b)); }
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) { return as_u(as_v(a) - as_v(b)); }
+__device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
+    return as_u(__builtin_elementwise_min(as_v(a), as_v(b)));
+}
+
+// byte <-> packed-u16 conversions (v_perm_b32)
+__device__ __forceinline__ uint32_t bytes01(uint32_t w) { return __builtin_amdgcn_perm(0u, w, 0x0C010C00u); }
+__device__ __forceinline__ uint32_t bytes23(uint32_t w) { return __builtin_amdgcn_perm(0u, w, 0x0C030C02u); }
+// low byte of each u16 of lo and hi -> [lo.0, lo.1, hi.0, hi.1] (uint8 truncation)
+__device__ __forceinline__ uint32_t pack4(uint32_t lo, uint32_t hi) {
+    return __builtin_amdgcn_perm(hi, lo, 0x06040200u);
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
+}
+constexpr int kRowShl1 = 0x101;
+constexpr int kRowShr1 = 0x111;
+
+// minimum over the 16 lanes of a DPP row; every lane receives it
+__device__ __forceinline__ uint32_t row_min16(uint32_t m) {
+    m = min(m, dpp<0x128>(0u, m));
+    m = min(m, dpp<0x124>(0u, m));
+    m = min(m, dpp<0x122>(0u, m));
+    m = min(m, dpp<0x121>(0u, m));
+    return m;
+}
+
+__device__ __forceinline__ int find_job(const int* __restrict__ offs, int n, int item) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (offs[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Global-address-space views so loads/stores compile to global_* (not flat_*).
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) uint32_t g_u32;
+typedef __attribute__((address_space(1))) u32x2 g_u2;
+typedef __attribute__((address_space(1))) u32x4 g_u4;
+
+template <int NW>
+__device__ __forceinline__ void load_words(const uint8_t* p, uint32_t (&w)[NW]) {
+    if constexpr (NW == 1) {
+        w[0] = *(const g_u32*)p;
+    } else if constexpr (NW == 2) {
+        const u32x2 v = *(const g_u2*)p;
+        w[0] = v.x; w[1] = v.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NW; k += 4) {
+            const u32x4 v = *(const g_u4*)(p + 4 * k);
+            w[k] = v.x; w[k + 1] = v.y; w[k + 2] = v.z; w[k + 3] = v.w;
+        }
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ void store_words(uint8_t* p, const uint32_t (&w)[NW]) {
+    if constexpr (NW == 1) {
+        *(g_u32*)p = w[0];
+    } else if constexpr (NW == 2) {
+        *(g_u2*)p = u32x2{w[0], w[1]};
+    } else {
+#pragma unroll
+        for (int k = 0; k < NW; k += 4)
+            *(g_u4*)(p + 4 * k) = u32x4{w[k], w[k + 1], w[k + 2], w[k + 3]};
+    }
+}
+
+// ---------------------------------------------------------------- census ---
+// cusgm_cc.cu:30-106.  Bit = (neighbour < centre), MSB-first raster order,
+// centre included; border pixels get 0 (left view) or 0xFFFFFFFF (right).
+template <int RW, int RH>
+__device__ __forceinline__ uint64_t census_px(const uint8_t (*t)[72], int r, int c) {
+    const uint8_t ctr = t[r][c];
+    uint64_t v = 0;
+#pragma unroll
+    for (int i = -RH; i <= RH; ++i)
+#pragma unroll
+        for (int j = -RW; j <= RW; ++j)
+            v = (v << 1) | (uint64_t)(t[r + i][c + j] < ctr);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ jobs,
+                                                     const int* __restrict__ offs, int njobs) {
+    __shared__ uint8_t tile[22][72];
+    const int item = blockIdx.x;
+    const int j = find_job(offs, njobs, item);
+    const SgmJob& J = jobs[j];
+    const int local = item - offs[j];
+    const int tiles_x = (J.W + 63) / 64;
+    const int x0 = (local % tiles_x) * 64, y0 = (local / tiles_x) * 16;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < 22 * 72; idx += 256) {
+        const int r = idx / 72, c = idx % 72;
+        const int yy = min(max(y0 - 3 + r, 0), J.H - 1);
+        const int xx = min(max(x0 - 4 + c, 0), J.W - 1);
+        tile[r][c] = J.img[(int64_t)yy * J.W + xx];
+    }
+    __syncthreads();
+    const int lx = tid & 63, ly = tid >> 6;
+    const int rw = J.cs_type ? 4 : 2, rh = J.cs_type ? 3 : 2;
+    const uint64_t invalid = J.is_right ? 0xFFFFFFFFull : 0ull;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        const int y = y0 + ly * 4 + rr, x = x0 + lx;
+        if (x >= J.W || y >= J.H) continue;
+        uint64_t v;
+        if (x < rw || x >= J.W - rw || y < rh || y >= J.H - rh)
+            v = invalid;
+        else if (J.cs_type)
+            v = census_px<4, 3>(tile, ly * 4 + rr + 3, lx + 4);
+        else
+            v = census_px<2, 2>(tile, ly * 4 + rr + 3, lx + 4);
+        J.census[(int64_t)y * J.W + x] = v;
+    }
+}
+
+// ------------------------------------------------------------------ cost ---
+// cusgm_cc.cu:118-172 (u64) / :238-293 (u32), launch :472-523.
+//   left view:  C[y][x][t] = popc(cl[x] ^ cr[x - min_d - t])
+//   right view: C[y][x][t] = popc(cr[x] ^ cl[x + min_d + t])
+// a partner column outside [0, W) uses column 0 of the partner census.
+// 5x5 right view: the reference loop skips the last pixel of every D-pixel
+// block (cusgm_cc.cu:288); its reused buffer still holds the left-view cost
+// of that pixel, which is what is written here.
+__global__ __launch_bounds__(256) void cost_kernel(const SgmJob* __restrict__ jobs,
+                                                   const int* __restrict__ offs, int njobs) {
+    __shared__ uint64_t sb[64];
+    __shared__ uint64_t sw[64 + kMaxD];
+    const int item = blockIdx.x;
+    const int j = find_job(offs, njobs, item);
+    const SgmJob& J = jobs[j];
+    const int local = item - offs[j];
+    const int W = J.W, D = J.D, min_d = J.min_d;
+    const int segs = (W - J.roi_x + 63) / 64;
+    const int y = local / segs;
+    const int x0 = J.roi_x + (local % segs) * 64;
+    const uint64_t* cb = J.census + (int64_t)y * W;
+    const uint64_t* cm = J.census_other + (int64_t)y * W;
+    const bool l2r = !J.is_right;
+    const int tid = threadIdx.x;
+    if (tid < 64) sb[tid] = (x0 + tid < W) ? cb[x0 + tid] : 0ull;
+    const int wbase = l2r ? x0 - min_d - (D - 1) : x0 + min_d;
+    const uint64_t cm0 = cm[0];
+    for (int k = tid; k < 64 + D - 1; k += 256) {
+        const int xm = wbase + k;
+        sw[k] = (xm >= 0 && xm < W) ? cm[xm] : cm0;
+    }
+    __syncthreads();
+    const int groups = D / 16;
+    for (int w = tid; w < 64 * groups; w += 256) {
+        const int i = w / groups, g = w - (w / groups) * groups;
+        const int x = x0 + i;
+        if (x >= W) continue;
+        const bool quirk = J.is_right && J.cs_type == 0 &&
+                           (((x - J.roi_x) % D) == D - 1 || x == W - 1);
+        uint32_t out[4] = {0u, 0u, 0u, 0u};
+        if (!quirk) {
+            const uint64_t b = sb[i];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int t = g * 16 + q;
+                const uint64_t m = l2r ? sw[i - t + D - 1] : sw[i + t];
+                out[q >> 2] |= (uint32_t)__popcll(b ^ m) << (8 * (q & 3));
+            }
+        } else {
+            const uint64_t b = cm[x];
+            const uint64_t cb0 = cb[0];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int xr = x - min_d - (g * 16 + q);
+                const uint64_t m = (xr >= 0 && xr < W) ? cb[xr] : cb0;
+                out[q >> 2] |= (uint32_t)__popcll(b ^ m) << (8 * (q & 3));
+            }
+        }
+        *reinterpret_cast<uint4*>(J.cost + ((int64_t)y * W + x) * D + g * 16) =
+            make_uint4(out[0], out[1], out[2], out[3]);
+    }
+}
+
+// ------------------------------------------------------------- aggregate ---
+// One SGM direction over four scanlines (cusgm_ca.cu:1480-1651 for D=64,
+// :1654-1888 for D=128, generic :156-226 etc. for D>=256), in exact integer
+// arithmetic:
+//   L(p0, d) = C(p0, d)                                   (stored, A8)
+//   L(p, d)  = C(p, d) + min(L'(d), B-(d), B+(d), m' + P2) - m'
+//   B-(d) = d == 0 ? 127 : L'(d-1) + P1,  B+(d) = d == D-1 ? 127 : L'(d+1) + P1
+//   m = min_d L(p, d); the stored value is L mod 256 (uint8 store).
+// All terms are < 2^16 when P1 + P2 <= 65000, so packed u16 is exact.
+template <int D>
+__device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, int lane) {
+    constexpr int DPD = D / 16;   // disparities per lane
+    constexpr int NW = DPD / 4;   // u32 words of cost per lane
+    constexpr int NP = DPD / 2;   // packed u16 pairs per lane
+    const int sub = lane >> 4, l = lane & 15;
+    const bool horiz = (path == kLR || path == kRL);
+    const bool fwd = (path == kUD || path == kLR);
+    const int W = J.W, H = J.H, roi_x = J.roi_x;
+    const int nlines = horiz ? H : (W - roi_x);
+    int line = group * 4 + sub;
+    const bool active = line < nlines;
+    if (!active) line = nlines - 1;
+    const int n = horiz ? (W - roi_x) : H;
+    int64_t x0, y0, step;
+    if (horiz) {
+        y0 = line; x0 = fwd ? roi_x : W - 1; step = fwd ? D : -D;
+    } else {
+        x0 = roi_x + line; y0 = fwd ? 0 : H - 1;
+        step = fwd ? (int64_t)W * D : -(int64_t)W * D;
+    }
+    const int64_t off0 = (y0 * W + x0) * D + l * DPD;
+    const uint8_t* const src0 = J.cost + off0;
+    uint8_t* const out_base = (path == 0) ? J.path[0] : (path == 1) ? J.path[1]
+                            : (path == 2) ? J.path[2] : J.path[3];
+    uint8_t* const dst0 = out_base + off0;
+    const uint32_t P1v = (uint32_t)J.p1 * 0x10001u;
+    const uint32_t P2v = (uint32_t)J.p2 * 0x10001u;
+
+    uint32_t w[NW];
+    load_words<NW>(src0, w);
+    uint32_t L[NP];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) { L[2 * k] = bytes01(w[k]); L[2 * k + 1] = bytes23(w[k]); }
+    if (active) store_words<NW>(dst0, w);
+    uint32_t t0 = L[0];
+#pragma unroll
+    for (int k = 1; k < NP; ++k) t0 = pk_min(t0, L[k]);
+    uint32_t m = row_min16(min(t0 & 0xFFFFu, t0 >> 16));
+
+    // one recurrence step: consumes the cost words of step i, updates L, m
+    auto advance = [&](const uint32_t (&c8)[NW], int i) __attribute__((always_inline)) {
+        const uint32_t mv = m * 0x10001u;
+        const uint32_t mp2 = pk_add(mv, P2v);
+        uint32_t R[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) R[k] = pk_add(L[k], P1v);
+        // neighbours across lanes; lane 0 / lane 15 of a row get the 127 term
+        const uint32_t prev_last = dpp<kRowShr1>(127u << 16, R[NP - 1]);
+        const uint32_t next_first = dpp<kRowShl1>(127u, R[0]);
+        uint32_t E[NP + 1];
+        E[0] = __builtin_amdgcn_alignbit(R[0], prev_last, 16);
+#pragma unroll
+        for (int k = 1; k < NP; ++k) E[k] = __builtin_amdgcn_alignbit(R[k], R[k - 1], 16);
+        E[NP] = __builtin_amdgcn_alignbit(next_first, R[NP - 1], 16);
+        uint32_t t = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const uint32_t c = (k & 1) ? bytes23(c8[k >> 1]) : bytes01(c8[k >> 1]);
+            const uint32_t b = pk_min(E[k], E[k + 1]);
+            const uint32_t v = pk_min(pk_min(L[k], b), mp2);
+            L[k] = pk_sub(pk_add(c, v), mv);
+            t = pk_min(t, L[k]);
+        }
+        uint32_t o[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) o[k] = pack4(L[2 * k], L[2 * k + 1]);
+        if (active) store_words<NW>(dst0 + (int64_t)i * step, o);
+        m = row_min16(min(t & 0xFFFFu, t >> 16));
+    };
+
+    // Register ring of prefetched cost words, about 64 bytes per lane in
+    // flight so a wave covers many steps of memory latency.  Slot p holds
+    // step base + p; refills are unconditional, clamped to the last step.
+    constexpr int PF = (NW >= 8) ? 2 : 16 / NW;
+    uint32_t buf[PF][NW];
+#pragma unroll
+    for (int p = 0; p < PF; ++p) load_words<NW>(src0 + (int64_t)min(1 + p, n - 1) * step, buf[p]);
+    int i = 1;
+    for (; i + PF <= n; i += PF) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+            uint32_t c8[NW];
+#pragma unroll
+            for (int k = 0; k < NW; ++k) c8[k] = buf[p][k];
+            load_words<NW>(src0 + (int64_t)min(i + p + PF, n - 1) * step, buf[p]);
+            advance(c8, i + p);
+        }
+    }
+    for (; i < n; ++i) {
+        uint32_t c8[NW];
+        load_words<NW>(src0 + (int64_t)i * step, c8);
+        advance(c8, i);
+    }
+}
+
+template <int DMAX>
+__global__ __launch_bounds__(64) void aggregate_kernel(const SgmJob* __restrict__ jobs,
+                                                       const int* __restrict__ offs, int njobs) {
+    const int item = blockIdx.x;
+    const int j = find_job(offs, njobs, item);
+    const SgmJob& J = jobs[j];
+    const int local = item - offs[j];
+    const int nh = (J.H + 3) / 4, nv = (J.W - J.roi_x + 3) / 4;
+    int path, group;
+    // long horizontal scanlines are issued first
+    if (local < nh) { path = kLR; group = local; }
+    else if (local < 2 * nh) { path = kRL; group = local - nh; }
+    else if (local < 2 * nh + nv) { path = kUD; group = local - 2 * nh; }
+    else { path = kDU; group = local - 2 * nh - nv; }
+    const int lane = threadIdx.x;
+    switch (J.D) {
+        case 64: agg_lines<64>(J, path, group, lane); break;
+        case 128: if constexpr (DMAX >= 128) agg_lines<128>(J, path, group, lane); break;
+        case 256: if constexpr (DMAX >= 256) agg_lines<256>(J, path, group, lane); break;
+        case 512: if constexpr (DMAX >= 512) agg_lines<512>(J, path, group, lane); break;
+        default: break;
+    }
+}
+
+// ------------------------------------------------------------------- wta ---
+// cusgm_ca.cu:3606-3699 (D=64), :3701-3809 (D=128), :3149-3232 (generic),
+// SubPixBias :150-154.  16 lanes per pixel, D/16 disparities per lane.
+// argmin ties -> smallest d (key = S << 16 | d).
+template <int D>
+__device__ __forceinline__ void wta_pixel(const SgmJob& J, int x, int y, int l,
+                                          uint16_t* __restrict__ s) {
+    constexpr int DPD = D / 16, NW = DPD / 4, NP = DPD / 2;
+    const bool inimg = x < J.W;
+    const int xc = inimg ? x : J.W - 1;
+    const int64_t off = ((int64_t)y * J.W + xc) * D + l * DPD;
+    uint32_t S[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) S[k] = 0u;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        uint32_t w[NW];
+        load_words<NW>(J.path[r] + off, w);
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            S[2 * k] = pk_add(S[2 * k], bytes01(w[k]));
+            S[2 * k + 1] = pk_add(S[2 * k + 1], bytes23(w[k]));
+        }
+    }
+    uint32_t best = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const uint32_t d0 = (uint32_t)(l * DPD + 2 * k);
+        best = min(best, (S[k] << 16) | d0);
+        best = min(best, (S[k] & 0xFFFF0000u) | (d0 + 1));
+    }
+    best = row_min16(best);
+    uint32_t* s32 = reinterpret_cast<uint32_t*>(s);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) s32[l * NP + k] = S[k];
+    __syncthreads();
+    if (l == 0 && inimg) {
+        const int W = J.W, H = J.H;
+        constexpr bool warp_rule = (D == 64 || D == 128);
+        const int rows_cov = warp_rule ? 2 * (H / 2) : H;
+        float outv = kInvalidDisp;
+        if (x >= J.roi_x && y < rows_cov && !(x > W - 5 || y > H - 5 || x < 5 || y < 5)) {
+            const uint32_t c2 = best >> 16;
+            const int ds = (int)(best & 0xFFFFu);
+            if (ds > 0 && (warp_rule || ds < D - 1)) {
+                const float c1 = (float)s[ds - 1];
+                const float c2f = (float)c2;
+                const float c3 = (ds == D - 1) ? (float)(D - 1) : (float)s[ds + 1];
+                const float a = c1 - c2f, b = c3 - c2f;
+                const float uniq = (a < b) ? a : b;
+                const float ut = 1.0f - J.uniqueness;
+                if (!(uniq <= c2f * ut)) {
+                    const float t = c1 + c3 - 2.0f * c2f;
+                    const float den = (1.0f < t) ? t : 1.0f;
+                    // correctly rounded single-precision quotient
+                    const float bias = (float)((double)(c1 - c3 + den) / (double)(2.0f * den));
+                    outv = (float)(ds + J.min_d) + bias;
+                }
+            }
+        }
+        J.disp[(int64_t)y * W + x] = outv;
+    }
+}
+
+template <int DMAX>
+__global__ __launch_bounds__(256) void wta_kernel(const SgmJob* __restrict__ jobs,
+                                                  const int* __restrict__ offs, int njobs) {
+    __shared__ __align__(16) uint16_t sS[16][DMAX];
+    const int item = blockIdx.x;
+    const int j = find_job(offs, njobs, item);
+    const SgmJob& J = jobs[j];
+    const int local = item - offs[j];
+    const int segs = (J.W + 15) / 16;
+    const int y = local / segs;
+    const int pix = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const int x = (local - y * segs) * 16 + pix;
+    switch (J.D) {
+        case 64: wta_pixel<64>(J, x, y, l, sS[pix]); break;
+        case 128: if constexpr (DMAX >= 128) wta_pixel<128>(J, x, y, l, sS[pix]); break;
+        case 256: if constexpr (DMAX >= 256) wta_pixel<256>(J, x, y, l, sS[pix]); break;
+        case 512: if constexpr (DMAX >= 512) wta_pixel<512>(J, x, y, l, sS[pix]); break;
+        default: break;
+    }
+}
+
+// ------------------------------------------------------------------ post ---
+// LR check (cusgm_ca.cu:3873-3898) evaluated per pixel against the right
+// map as it was before the check, then the 3x3 median (cusgm_df.cu:16-75)
+// out of place; x = 0 / y = 0 pass through, outside neighbours are INVALID.
+__device__ __forceinline__ float lr_value(const SgmPost& P, int x, int y) {
+    const float dl = P.raw_l[(int64_t)y * P.W + x];
+    if (!P.do_lr || dl == kInvalidDisp) return dl;
+    const float xrf = (float)x - dl;
+    const int xr = (xrf > 0) ? (int)((double)xrf + 0.5) : (int)((double)xrf - 0.5);
+    if (xr < 0 || xr >= P.W) return kInvalidDisp;
+    const float dr = P.raw_r[(int64_t)y * P.W + xr];
+    if (fabsf(dl - dr) > P.lr_thr) return kInvalidDisp;
+    return (dl + dr) / 2.0f;
+}
+
+__device__ __forceinline__ float median9(float a0, float a1, float a2, float b0, float b1,
+                                         float b2, float c0, float c1, float c2) {
+    // sort each column (a_i, b_i, c_i); the median of the nine values is
+    // med3(max of the column minima, med of the column medians, min of the maxima)
+    const float lo0 = fminf(fminf(a0, b0), c0), hi0 = fmaxf(fmaxf(a0, b0), c0);
+    const float lo1 = fminf(fminf(a1, b1), c1), hi1 = fmaxf(fmaxf(a1, b1), c1);
+    const float lo2 = fminf(fminf(a2, b2), c2), hi2 = fmaxf(fmaxf(a2, b2), c2);
+    const float md0 = __builtin_amdgcn_fmed3f(a0, b0, c0);
+    const float md1 = __builtin_amdgcn_fmed3f(a1, b1, c1);
+    const float md2 = __builtin_amdgcn_fmed3f(a2, b2, c2);
+    const float p = fmaxf(fmaxf(lo0, lo1), lo2);
+    const float q = __builtin_amdgcn_fmed3f(md0, md1, md2);
+    const float r = fminf(fminf(hi0, hi1), hi2);
+    return __builtin_amdgcn_fmed3f(p, q, r);
+}
+
+__global__ __launch_bounds__(256) void post_kernel(const SgmPost* __restrict__ posts,
+                                                   const int* __restrict__ offs, int nposts) {
+    __shared__ float t[18][18];
+    const int item = blockIdx.x;
+    const int j = find_job(offs, nposts, item);
+    const SgmPost& P = posts[j];
+    const int local = item - offs[j];
+    const int tiles_x = (P.W + 15) / 16;
+    const int x0 = (local % tiles_x) * 16, y0 = (local / tiles_x) * 16;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < 18 * 18; idx += 256) {
+        const int r = idx / 18, c = idx % 18;
+        const int yy = y0 - 1 + r, xx = x0 - 1 + c;
+        t[r][c] = (xx >= 0 && yy >= 0 && xx < P.W && yy < P.H) ? lr_value(P, xx, yy) : kInvalidDisp;
+    }
+    __syncthreads();
+    const int tx = tid & 15, ty = tid >> 4;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= P.W || y >= P.H) return;
+    float v = t[ty + 1][tx + 1];
+    if (P.do_median && x > 0 && y > 0)
+        v = median9(t[ty][tx], t[ty][tx + 1], t[ty][tx + 2], t[ty + 1][tx], t[ty + 1][tx + 1],
+                    t[ty + 1][tx + 2], t[ty + 2][tx], t[ty + 2][tx + 1], t[ty + 2][tx + 2]);
+    P.out_l[(int64_t)y * P.W + x] = v;
+}
+
+// Right-map invalidation of the LR check (cusgm_ca.cu:3894-3895): every left
+// pixel that fails marks its partner INVALID in out_r, which holds a copy of
+// the right map.  All writers store the same value, so the result is fixed.
+__global__ __launch_bounds__(256) void right_invalidate_kernel(const SgmPost* __restrict__ posts,
+                                                               const int* __restrict__ offs,
+                                                               int nposts) {
+    const int item = blockIdx.x;
+    const int j = find_job(offs, nposts, item);
+    const SgmPost& P = posts[j];
+    if (!P.out_r || !P.do_lr) return;
+    const int local = item - offs[j];
+    const int tiles_x = (P.W + 15) / 16;
+    const int x = (local % tiles_x) * 16 + (threadIdx.x & 15);
+    const int y = (local / tiles_x) * 16 + (threadIdx.x >> 4);
+    if (x >= P.W || y >= P.H) return;
+    const float dl = P.raw_l[(int64_t)y * P.W + x];
+    if (dl == kInvalidDisp) return;
+    const float xrf = (float)x - dl;
+    const int xr = (xrf > 0) ? (int)((double)xrf + 0.5) : (int)((double)xrf - 0.5);
+    if (xr < 0 || xr >= P.W) return;
+    const float dr = P.raw_r[(int64_t)y * P.W + xr];
+    if (fabsf(dl - dr) > P.lr_thr) P.out_r[(int64_t)y * P.W + xr] = kInvalidDisp;
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ dst, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = v;
+}
+
+}  // namespace
+
+hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, s, dst, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_census(const StagePlan& p, hipStream_t s) {
+    if (p.n_census <= 0) return hipSuccess;
+    hipLaunchKernelGGL(census_kernel, dim3(p.n_census), dim3(256), 0, s, p.jobs, p.offs_census,
+                       p.njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_cost(const StagePlan& p, hipStream_t s) {
+    if (p.n_cost <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cost_kernel, dim3(p.n_cost), dim3(256), 0, s, p.jobs, p.offs_cost, p.njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_aggregate(const StagePlan& p, hipStream_t s) {
+    if (p.n_agg <= 0) return hipSuccess;
+    const dim3 g(p.n_agg), b(64);
+    if (p.dmax <= 64)
+        hipLaunchKernelGGL(aggregate_kernel<64>, g, b, 0, s, p.jobs, p.offs_agg, p.njobs);
+    else if (p.dmax <= 128)
+        hipLaunchKernelGGL(aggregate_kernel<128>, g, b, 0, s, p.jobs, p.offs_agg, p.njobs);
+    else if (p.dmax <= 256)
+        hipLaunchKernelGGL(aggregate_kernel<256>, g, b, 0, s, p.jobs, p.offs_agg, p.njobs);
+    else
+        hipLaunchKernelGGL(aggregate_kernel<512>, g, b, 0, s, p.jobs, p.offs_agg, p.njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_wta(const StagePlan& p, hipStream_t s) {
+    if (p.n_wta <= 0) return hipSuccess;
+    const dim3 g(p.n_wta), b(256);
+    if (p.dmax <= 64)
+        hipLaunchKernelGGL(wta_kernel<64>, g, b, 0, s, p.jobs, p.offs_wta, p.njobs);
+    else if (p.dmax <= 128)
+        hipLaunchKernelGGL(wta_kernel<128>, g, b, 0, s, p.jobs, p.offs_wta, p.njobs);
+    else if (p.dmax <= 256)
+        hipLaunchKernelGGL(wta_kernel<256>, g, b, 0, s, p.jobs, p.offs_wta, p.njobs);
+    else
+        hipLaunchKernelGGL(wta_kernel<512>, g, b, 0, s, p.jobs, p.offs_wta, p.njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_post(const StagePlan& p, hipStream_t s) {
+    if (p.n_post <= 0) return hipSuccess;
+    hipLaunchKernelGGL(post_kernel, dim3(p.n_post), dim3(256), 0, s, p.posts, p.offs_post,
+                       p.nposts);
+    return hipGetLastError();
+}
+
+hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s) {
+    if (p.n_post <= 0) return hipSuccess;
+    hipLaunchKernelGGL(right_invalidate_kernel, dim3(p.n_post), dim3(256), 0, s, p.posts,
+                       p.offs_post, p.nposts);
+    return hipGetLastError();
+}
+
+}  // namespace fsgm

@@ -1,0 +1,102 @@
+// stereo_demo: drives the StereoCuda drop-in class exactly as a caller of the
+// reference library would (Init, Match, Release), on raw files.
+//
+//   stereo_demo LEFT.u8 RIGHT.u8 W H MIN_D RANGE OUT_LEFT.f32 [OUT_RIGHT.f32]
+//               [--cs 0|1] [--p1 N] [--p2 N] [--uniq F] [--lr 0|1] [--lrthr F]
+//               [--median 0|1] [--repeat N]
+//
+// Inputs are tightly packed uint8 W x H; outputs tightly packed float32.
+// Prints the average Match time over --repeat calls after the first.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "StereoCuda.h"
+
+static bool read_file(const char* path, std::vector<uint8>& buf, size_t n) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    buf.resize(n);
+    const size_t got = std::fread(buf.data(), 1, n, f);
+    std::fclose(f);
+    return got == n;
+}
+
+static bool write_file(const char* path, const void* data, size_t n) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return false;
+    const size_t put = std::fwrite(data, 1, n, f);
+    std::fclose(f);
+    return put == n;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 8) {
+        std::fprintf(stderr, "usage: %s LEFT RIGHT W H MIN_D RANGE OUT_LEFT [OUT_RIGHT] [options]\n",
+                     argv[0]);
+        return 2;
+    }
+    const int W = std::atoi(argv[3]), H = std::atoi(argv[4]);
+    const int min_d = std::atoi(argv[5]), range = std::atoi(argv[6]);
+    const char* out_left = argv[7];
+    const char* out_right = nullptr;
+    int a = 8;
+    if (a < argc && std::strncmp(argv[a], "--", 2) != 0) out_right = argv[a++];
+
+    CuSGMOption opt;
+    opt.cs_type = CuSGMOption::CS_9x7;
+    opt.p1 = 10;
+    opt.p2_init = 150;
+    opt.unique_threshold = 0.95f;
+    opt.do_lr_check = true;
+    opt.lr_threshold = 1.0f;
+    opt.do_median_filter = true;
+    opt.using_constant_p2 = true;
+    int repeat = 1;
+    for (; a + 1 < argc; a += 2) {
+        const std::string k = argv[a];
+        const char* v = argv[a + 1];
+        if (k == "--cs") opt.cs_type = std::atoi(v) ? CuSGMOption::CS_9x7 : CuSGMOption::CS_5x5;
+        else if (k == "--p1") opt.p1 = std::atoi(v);
+        else if (k == "--p2") opt.p2_init = std::atoi(v);
+        else if (k == "--uniq") opt.unique_threshold = (float)std::atof(v);
+        else if (k == "--lr") opt.do_lr_check = std::atoi(v) != 0;
+        else if (k == "--lrthr") opt.lr_threshold = (float)std::atof(v);
+        else if (k == "--median") opt.do_median_filter = std::atoi(v) != 0;
+        else if (k == "--repeat") repeat = std::atoi(v);
+        else { std::fprintf(stderr, "unknown option %s\n", k.c_str()); return 2; }
+    }
+
+    std::vector<uint8> left, right;
+    const size_t npix = (size_t)W * H;
+    if (!read_file(argv[1], left, npix) || !read_file(argv[2], right, npix)) {
+        std::fprintf(stderr, "cannot read the input images\n");
+        return 2;
+    }
+    std::vector<float32> dl(npix), dr(npix);
+
+    StereoCuda stereo;
+    if (!stereo.Init(W, H, min_d, range, opt, false)) {
+        std::fprintf(stderr, "Init failed: %s\n", stereo.LastError());
+        return 1;
+    }
+    double total_ms = 0.0;
+    for (int r = 0; r < repeat; ++r) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!stereo.Match(left.data(), right.data(), dl.data(), out_right ? dr.data() : nullptr)) {
+            std::fprintf(stderr, "Match failed: %s\n", stereo.LastError());
+            return 1;
+        }
+        const auto t1 = std::chrono::steady_clock::now();
+        if (r > 0 || repeat == 1) total_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
+    stereo.Release();
+    const int timed = repeat > 1 ? repeat - 1 : 1;
+    std::printf("match_ms %.4f\n", total_ms / timed);
+    if (!write_file(out_left, dl.data(), npix * 4)) return 1;
+    if (out_right && !write_file(out_right, dr.data(), npix * 4)) return 1;
+    return 0;
+}

@@ -1,0 +1,76 @@
+"""Stage-level entry points on device tensors (for parity checks).
+
+Each function runs one stage of the plain pipeline through the C ABI
+(sgm_stage_*), on torch tensors resident on the GPU, and returns new
+tensors.  Layouts: census int64 [H, W] (bit pattern of the uint64 census),
+cost / path volumes uint8 [H, W, D], disparity float32 [H, W].
+"""
+from __future__ import annotations
+
+import torch
+
+from ._lib import PP4, check, lib
+
+DIRS = ("top->bottom", "left->right", "bottom->top", "right->left")
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _dev(t: torch.Tensor, dtype) -> torch.Tensor:
+    if not t.is_cuda or t.dtype != dtype or not t.is_contiguous():
+        raise ValueError(f"expected a contiguous {dtype} CUDA tensor")
+    return t
+
+
+def census(img: torch.Tensor, cs_type: int, is_right: bool) -> torch.Tensor:
+    _dev(img, torch.uint8)
+    H, W = img.shape
+    out = torch.empty((H, W), dtype=torch.int64, device=img.device)
+    check(lib().sgm_stage_census(img.data_ptr(), W, H, cs_type, int(is_right), out.data_ptr(),
+                                 _stream()))
+    return out
+
+
+def cost(census_base: torch.Tensor, census_match: torch.Tensor, min_d: int, D: int,
+         left_to_right: bool, cs_type: int) -> torch.Tensor:
+    _dev(census_base, torch.int64)
+    _dev(census_match, torch.int64)
+    H, W = census_base.shape
+    out = torch.zeros((H, W, D), dtype=torch.uint8, device=census_base.device)
+    check(lib().sgm_stage_cost(census_base.data_ptr(), census_match.data_ptr(), W, H, min_d, D,
+                               int(left_to_right), cs_type, out.data_ptr(), _stream()))
+    return out
+
+
+def aggregate(cost_vol: torch.Tensor, min_d: int, p1: int, p2: int):
+    _dev(cost_vol, torch.uint8)
+    H, W, D = cost_vol.shape
+    paths = [torch.zeros_like(cost_vol) for _ in range(4)]
+    ptrs = PP4(*[p.data_ptr() for p in paths])
+    check(lib().sgm_stage_aggregate(cost_vol.data_ptr(), W, H, min_d, D, p1, p2, ptrs, _stream()))
+    return paths
+
+
+def wta(paths, min_d: int, uniqueness: float) -> torch.Tensor:
+    for p in paths:
+        _dev(p, torch.uint8)
+    H, W, D = paths[0].shape
+    out = torch.empty((H, W), dtype=torch.float32, device=paths[0].device)
+    ptrs = PP4(*[p.data_ptr() for p in paths])
+    check(lib().sgm_stage_wta(ptrs, W, H, min_d, D, uniqueness, out.data_ptr(), _stream()))
+    return out
+
+
+def post(raw_left: torch.Tensor, raw_right: torch.Tensor | None, do_lr: bool, lr_threshold: float,
+         do_median: bool, want_right: bool = False):
+    _dev(raw_left, torch.float32)
+    H, W = raw_left.shape
+    out_l = torch.empty_like(raw_left)
+    out_r = torch.empty_like(raw_left) if want_right else None
+    check(lib().sgm_stage_post(raw_left.data_ptr(),
+                               None if raw_right is None else _dev(raw_right, torch.float32).data_ptr(),
+                               W, H, int(do_lr), lr_threshold, int(do_median), out_l.data_ptr(),
+                               None if out_r is None else out_r.data_ptr(), _stream()))
+    return out_l, out_r

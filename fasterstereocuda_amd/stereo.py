@@ -1,0 +1,144 @@
+"""Python mirror of the StereoCuda drop-in class.
+
+Same method names, argument meaning and bool-return convention as the
+reference's SgmStereoCuda/StereoCuda.h:14-100; every call goes through the
+C ABI of libfsgm.so.  Host buffers are numpy arrays; ``match_device`` takes
+device tensors (anything with ``data_ptr()``) and the current stream.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import SgmOption, SgmRoi, check, lib
+
+
+@dataclass
+class CuSGMOption:
+    """Same fields and defaults as CuSGMOption (sgm_types.h:21-65)."""
+    p1: int = 8
+    p2_init: int = 32
+    p2_max: int = 0
+    cs_type: int = 0            # 0 = CS_5x5, 1 = CS_9x7
+    num_paths: int = 0
+    do_median_filter: bool = False
+    do_lr_check: bool = False
+    lr_threshold: float = 0.0
+    unique_threshold: float = 0.0
+    using_constant_p2: bool = False
+    do_remove_peaks: bool = False
+    peaks_ratio_threshold: float = 0.0
+    post_filter_type: int = 0
+    morphology_type: int = 0
+
+    CS_5x5 = 0
+    CS_9x7 = 1
+
+    def to_c(self) -> SgmOption:
+        return SgmOption(int(self.p1), int(self.p2_init), int(self.p2_max), int(self.cs_type),
+                         int(self.num_paths), int(bool(self.do_median_filter)),
+                         int(bool(self.do_lr_check)), float(self.lr_threshold),
+                         float(self.unique_threshold), int(bool(self.using_constant_p2)),
+                         int(bool(self.do_remove_peaks)), float(self.peaks_ratio_threshold),
+                         int(self.post_filter_type), int(self.morphology_type))
+
+    @classmethod
+    def benchmark(cls, **kw) -> "CuSGMOption":
+        """The options of the benchmark workloads: 9x7 census, P1=10, P2=150,
+        uniqueness 0.95, LR check (tolerance 1) and 3x3 median."""
+        o = cls(p1=10, p2_init=150, cs_type=1, do_median_filter=True, do_lr_check=True,
+                lr_threshold=1.0, unique_threshold=0.95, using_constant_p2=True)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+class StereoCuda:
+    """Single-level SGM matcher (StereoCuda.h)."""
+
+    def __init__(self):
+        self._h = lib().sgm_create()
+        if not self._h:
+            raise MemoryError("sgm_create failed")
+        self.width = self.height = 0
+        self.last_error = ""
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().sgm_destroy(h)
+            self._h = None
+
+    def _ok(self, rc: int) -> bool:
+        if rc != _lib.SGM_OK:
+            self.last_error = _lib.last_error(self._h)
+            return False
+        return True
+
+    def Init(self, width: int, height: int, min_disparity: int, disp_range: int,
+             sgm_option: CuSGMOption, print_log: bool = False) -> bool:
+        opt = sgm_option.to_c()
+        ok = self._ok(lib().sgm_init(self._h, width, height, min_disparity, disp_range,
+                                     ctypes.byref(opt), int(print_log)))
+        if ok:
+            self.width, self.height = width, height
+        return ok
+
+    def Match(self, img_left: np.ndarray, img_right: np.ndarray, disp_left: np.ndarray,
+              disp_right: np.ndarray | None = None, init_disp_left=None,
+              ste_roi_left=None, ste_roi_right=None) -> bool:
+        shape = (self.height, self.width)
+        for a, dt in ((img_left, np.uint8), (img_right, np.uint8), (disp_left, np.float32)):
+            if a.shape != shape or a.dtype != dt or not a.flags["C_CONTIGUOUS"]:
+                raise ValueError(f"expected C-contiguous {dt.__name__} array of shape {shape}")
+        if disp_right is not None and (disp_right.shape != shape or disp_right.dtype != np.float32
+                                       or not disp_right.flags["C_CONTIGUOUS"]):
+            raise ValueError("disp_right must be a C-contiguous float32 array of the image shape")
+        ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+        init = None if init_disp_left is None else init_disp_left.ctypes.data
+        rl = None if ste_roi_left is None else ctypes.byref(ste_roi_left)
+        rr = None if ste_roi_right is None else ctypes.byref(ste_roi_right)
+        return self._ok(lib().sgm_match(self._h, ptr(img_left), ptr(img_right), ptr(disp_left),
+                                        ptr(disp_right), init, rl, rr))
+
+    def match_device(self, d_left, d_right, d_disp_left, d_disp_right=None, stream=None) -> bool:
+        """Pipeline on device tensors (uint8 / float32, C-contiguous), enqueued
+        on `stream` (a hipStream_t as int, default: torch's current stream)."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        return self._ok(lib().sgm_match_device(
+            self._h, d_left.data_ptr(), d_right.data_ptr(), d_disp_left.data_ptr(),
+            None if d_disp_right is None else d_disp_right.data_ptr(), stream))
+
+    def SetMinDisparity(self, min_disparity: int) -> None:
+        self._ok(lib().sgm_set_min_disparity(self._h, min_disparity))
+
+    def GetDispartyRange(self) -> int:
+        return lib().sgm_get_disparity_range(self._h)
+
+    def Release(self) -> None:
+        lib().sgm_release(self._h)
+
+    @staticmethod
+    def GetRoiFromDispMap(disp: np.ndarray) -> SgmRoi:
+        roi = SgmRoi()
+        h, w = disp.shape
+        d = np.ascontiguousarray(disp, dtype=np.float32)
+        lib().sgm_roi_from_disparity(d.ctypes.data, w, h, ctypes.byref(roi))
+        return roi
+
+    @staticmethod
+    def get_invad_float() -> float:
+        return lib().sgm_invalid_float()
+
+    @staticmethod
+    def get_invad_short() -> int:
+        return lib().sgm_invalid_short()
+
+    @staticmethod
+    def get_level_range() -> int:
+        return lib().sgm_level_range()

@@ -1,0 +1,144 @@
+/*
+ * sgm_capi.h -- C ABI of the MI355X SGM stereo engine (libfsgm.so).
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, no C++ or torch
+ * types.  The C++ drop-in classes (StereoCuda.h, FasterStereoCuda.h) and the
+ * Python bindings call only these functions.
+ *
+ * Conventions
+ *   - return value: SGM_OK (0) or a negative SGM_E_* code; the message of
+ *     the last failure on a handle is in sgm_last_error(handle) (and
+ *     sgm_last_error(NULL) for calls without a handle).
+ *   - images are tightly packed row-major uint8 W x H; disparity maps are
+ *     tightly packed float32 W x H with invalid = -999999.0f.
+ *   - "device" entry points take device pointers and a hipStream_t passed as
+ *     void* (NULL = the null stream) and return after enqueueing work.
+ *   - a handle is not thread safe; it belongs to the device that was current
+ *     when sgm_init ran.
+ */
+#ifndef SGM_CAPI_H_
+#define SGM_CAPI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGM_OK               0
+#define SGM_E_INVALID_ARG   (-1)   /* bad size, pointer or option value */
+#define SGM_E_HIP           (-2)   /* a HIP runtime call failed */
+#define SGM_E_NOT_INIT      (-3)   /* handle used before a successful init */
+#define SGM_E_UNSUPPORTED   (-4)   /* option outside the built scope */
+#define SGM_E_NO_DEVICE     (-5)   /* no HIP device */
+
+#define SGM_INVALID_FLOAT   (-999999.0f)
+#define SGM_INVALID_SHORT   (-32768)
+#define SGM_LEVEL_RANGE     32
+
+/* Mirrors CuSGMOption field by field (SgmStereoCuda/sgm_types.h:21-65);
+ * enums are int32, bools are int32 (0/1). */
+typedef struct sgm_option {
+    int32_t p1;
+    int32_t p2_init;
+    int32_t p2_max;                 /* read by nobody, as in the reference */
+    int32_t cs_type;                /* 0 = 5x5 census, 1 = 9x7 census */
+    int32_t num_paths;              /* read by nobody; 4 paths are used */
+    int32_t do_median_filter;
+    int32_t do_lr_check;
+    float   lr_threshold;
+    float   unique_threshold;
+    int32_t using_constant_p2;      /* only constant P2 is built */
+    int32_t do_remove_peaks;        /* must be 0 (not built) */
+    float   peaks_ratio_threshold;
+    int32_t post_filter_type;       /* must be 0 = PF_NONE */
+    int32_t morphology_type;        /* must be 0 = MP_NONE */
+} sgm_option;
+
+/* StereoROI_T (sgm_types.h:89-92). */
+typedef struct sgm_roi { int32_t x, y, w, h; } sgm_roi;
+
+typedef struct sgm_handle sgm_handle;
+
+/* Replaces `new StereoCudaImpl` (SgmStereoCuda/StereoCuda.cpp:6-9). */
+sgm_handle* sgm_create(void);
+/* Replaces ~StereoCuda (StereoCuda.cpp:11-16): releases and frees. */
+void sgm_destroy(sgm_handle* h);
+
+/* Replaces StereoCuda::Init (StereoCuda.h:31, StereoCudaImpl.cpp:128-261).
+ * Arguments are validated before the device is touched.  disp_range must be
+ * 64 * 2^n with n in 0..3; P1 and P2 must be >= 0 with P1 + P2 <= 65000. */
+int sgm_init(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity,
+             int32_t disp_range, const sgm_option* opt, int32_t print_log);
+
+/* Replaces StereoCuda::Match (StereoCuda.h:44, StereoCudaImpl.cpp:507-546):
+ * host pointers, blocking.  disp_right may be NULL.  init_disp_left and the
+ * ROIs must be NULL (initial disparities are reached through the pyramid
+ * entry points; ROIs are not built). */
+int sgm_match(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
+              float* disp_left, float* disp_right, const int16_t* init_disp_left,
+              const sgm_roi* roi_left, const sgm_roi* roi_right);
+
+/* Same pipeline on device-resident buffers, enqueued on `stream`; no host
+ * synchronisation.  disp_right may be NULL. */
+int sgm_match_device(sgm_handle* h, const uint8_t* d_left, const uint8_t* d_right,
+                     float* d_disp_left, float* d_disp_right, void* stream);
+
+/* Replaces StereoCuda::SetMinDisparity / GetDispartyRange (StereoCuda.h:71-72). */
+int sgm_set_min_disparity(sgm_handle* h, int32_t min_disparity);
+int32_t sgm_get_disparity_range(const sgm_handle* h);
+
+/* Replaces StereoCuda::Release (StereoCuda.h:75).  Safe to call twice. */
+int sgm_release(sgm_handle* h);
+
+const char* sgm_last_error(const sgm_handle* h);
+
+/* Replaces StereoCuda::MallocPageLockedPtr / FreePageLockedPtr
+ * (StereoCuda.h:94-97). */
+int sgm_host_alloc(void** ptr, size_t size);
+int sgm_host_free(void* ptr);
+
+/* Replaces StereoCuda::GetRoiFromDispMap (StereoCuda.h:84,
+ * StereoCudaImpl.cpp:585-628): host-side scan. */
+void sgm_roi_from_disparity(const float* disp, int32_t width, int32_t height, sgm_roi* roi);
+
+/* StereoCuda::get_invad_float / get_invad_short / get_level_range
+ * (StereoCuda.h:87-91). */
+float sgm_invalid_float(void);
+int16_t sgm_invalid_short(void);
+int16_t sgm_level_range(void);
+
+/* ---- stage entry points (device pointers; used by the parity tests) ----
+ * Each runs one stage of the plain pipeline over one view.  Layouts:
+ * census u64 [H][W]; cost and path volumes u8 [H][W][D] (d fastest). */
+
+/* cusgm_cc.cu:30-106. */
+int sgm_stage_census(const uint8_t* d_img, int32_t width, int32_t height, int32_t cs_type,
+                     int32_t is_right, uint64_t* d_census, void* stream);
+/* cusgm_cc.cu:118-172, 238-293.  For the 5x5 right-to-left cost the pixels
+ * the reference kernel skips (cusgm_cc.cu:288) get the left-to-right cost
+ * its reused buffer held, recomputed from the two census maps. */
+int sgm_stage_cost(const uint64_t* d_census_base, const uint64_t* d_census_match,
+                   int32_t width, int32_t height, int32_t min_disparity, int32_t disp_range,
+                   int32_t left_to_right, int32_t cs_type, uint8_t* d_cost, void* stream);
+/* cusgm_ca.cu:1480-1888 and the generic kernels; d_paths[r] for r = 0 top->bottom,
+ * 1 left->right, 2 bottom->top, 3 right->left. */
+int sgm_stage_aggregate(const uint8_t* d_cost, int32_t width, int32_t height,
+                        int32_t min_disparity, int32_t disp_range, int32_t p1, int32_t p2,
+                        uint8_t* const d_paths[4], void* stream);
+/* cusgm_ca.cu:3149-3232, 3606-3809. */
+int sgm_stage_wta(uint8_t* const d_paths[4], int32_t width, int32_t height,
+                  int32_t min_disparity, int32_t disp_range, float uniqueness,
+                  float* d_disp, void* stream);
+/* cusgm_ca.cu:3873-3898 and cusgm_df.cu:16-75: LR check (if do_lr_check)
+ * then 3x3 median (if do_median); d_out_right (may be NULL) receives the
+ * right map with the LR invalidations applied. */
+int sgm_stage_post(const float* d_raw_left, const float* d_raw_right, int32_t width,
+                   int32_t height, int32_t do_lr_check, float lr_threshold,
+                   int32_t do_median, float* d_out_left, float* d_out_right, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGM_CAPI_H_ */

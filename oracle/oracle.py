@@ -1,0 +1,160 @@
+"""ctypes loader for the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module.  It wraps ``oracle/libsgm_oracle.so`` (built
+from ``oracle/sgm_oracle.cpp`` by ``oracle/Makefile``), a C++/OpenMP
+restatement of the reference SGM arithmetic; see the header of that file for
+the reference lines each function follows.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libsgm_oracle.so")
+_lib = None
+
+INVALID = np.float32(-999999.0)
+DIR_UD, DIR_LR, DIR_DU, DIR_RL = 0, 1, 2, 3
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("cs_type", ctypes.c_int32),
+        ("p1", ctypes.c_int32),
+        ("p2", ctypes.c_int32),
+        ("uniqueness", ctypes.c_float),
+        ("do_lr_check", ctypes.c_int32),
+        ("lr_threshold", ctypes.c_float),
+        ("do_median", ctypes.c_int32),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++ -fopenmp)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        F = ctypes.c_float
+        L.orc_num_threads.restype = I
+        L.orc_census.argtypes = [P, I, I, I, I, P]
+        L.orc_cost.argtypes = [P, P, I, I, I, I, I, I, P, P]
+        L.orc_aggregate.argtypes = [P, I, I, I, I, I, I, I, P]
+        L.orc_wta.argtypes = [P, P, P, P, I, I, I, I, I, F, P]
+        L.orc_lrcheck.argtypes = [P, P, P, I, I, F]
+        L.orc_median3x3.argtypes = [P, P, I, I]
+        L.orc_match.argtypes = [P, P, I, I, I, I, ctypes.POINTER(Options), P, P]
+        L.orc_match.restype = I
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def census(img: np.ndarray, cs_type: int, is_right: bool) -> np.ndarray:
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    H, W = img.shape
+    out = np.empty((H, W), np.uint64)
+    lib().orc_census(_p(img), W, H, cs_type, int(is_right), _p(out))
+    return out
+
+
+def cost(cs_base, cs_match, min_d: int, D: int, left_to_right: bool, cs_type: int,
+         l2r_cost: np.ndarray | None = None, out: np.ndarray | None = None) -> np.ndarray:
+    H, W = cs_base.shape
+    if out is None:
+        out = np.zeros((H, W, D), np.uint8)
+    lib().orc_cost(_p(cs_base), _p(cs_match), W, H, min_d, D, int(left_to_right), cs_type,
+                   _p(l2r_cost), _p(out))
+    return out
+
+
+def aggregate(cost_vol: np.ndarray, roi_x: int, direction: int, p1: int, p2: int) -> np.ndarray:
+    H, W, D = cost_vol.shape
+    out = np.zeros((H, W, D), np.uint8)
+    lib().orc_aggregate(_p(cost_vol), W, H, D, roi_x, direction, p1, p2, _p(out))
+    return out
+
+
+def wta(paths, min_d: int, roi_x: int, uniqueness: float) -> np.ndarray:
+    H, W, D = paths[0].shape
+    out = np.empty((H, W), np.float32)
+    lib().orc_wta(*[_p(p) for p in paths], W, H, D, min_d, roi_x, uniqueness, _p(out))
+    return out
+
+
+def lrcheck(disp_l: np.ndarray, disp_r: np.ndarray, threshold: float):
+    """Returns (checked left map, right map with invalidations applied)."""
+    H, W = disp_l.shape
+    left = disp_l.copy()
+    right_out = disp_r.copy()
+    lib().orc_lrcheck(_p(left), _p(np.ascontiguousarray(disp_r)), _p(right_out), W, H, threshold)
+    return left, right_out
+
+
+def median3x3(disp: np.ndarray) -> np.ndarray:
+    H, W = disp.shape
+    out = np.empty_like(disp)
+    lib().orc_median3x3(_p(np.ascontiguousarray(disp)), _p(out), W, H)
+    return out
+
+
+def match(left: np.ndarray, right: np.ndarray, min_d: int, D: int, cs_type: int = 1,
+          p1: int = 10, p2: int = 150, uniqueness: float = 0.95, do_lr_check: bool = True,
+          lr_threshold: float = 1.0, do_median: bool = True):
+    """Whole plain pipeline.  Returns (disp_left, disp_right)."""
+    left = np.ascontiguousarray(left, dtype=np.uint8)
+    right = np.ascontiguousarray(right, dtype=np.uint8)
+    H, W = left.shape
+    opt = Options(cs_type, p1, p2, uniqueness, int(do_lr_check), lr_threshold, int(do_median))
+    dl = np.empty((H, W), np.float32)
+    dr = np.empty((H, W), np.float32)
+    rc = lib().orc_match(_p(left), _p(right), W, H, min_d, D, ctypes.byref(opt), _p(dl), _p(dr))
+    if rc != 0:
+        raise RuntimeError(f"orc_match failed with {rc}")
+    return dl, dr
+
+
+def stages(left: np.ndarray, right: np.ndarray, min_d: int, D: int, cs_type: int = 1,
+           p1: int = 10, p2: int = 150, uniqueness: float = 0.95, lr_threshold: float = 1.0):
+    """Every intermediate of the plain pipeline with the LR check on, for
+    stage-by-stage parity checks.  Returns a dict of numpy arrays."""
+    roi_x = max(0, min_d)
+    out = {}
+    out["census_l"] = cl = census(left, cs_type, False)
+    out["census_r"] = cr = census(right, cs_type, True)
+    out["cost_l"] = c_l = cost(cl, cr, min_d, D, True, cs_type)
+    out["paths_l"] = pl = [aggregate(c_l, roi_x, r, p1, p2) for r in range(4)]
+    out["wta_l"] = wl = wta(pl, min_d, roi_x, uniqueness)
+    out["cost_r"] = c_r = cost(cr, cl, min_d, D, False, cs_type,
+                               l2r_cost=c_l if cs_type == 0 else None,
+                               out=c_l.copy())
+    out["paths_r"] = pr = [aggregate(c_r, roi_x, r, p1, p2) for r in range(4)]
+    out["wta_r"] = wr = wta(pr, min_d, roi_x, uniqueness)
+    out["lr_l"], out["lr_r"] = lrcheck(wl, wr, lr_threshold)
+    out["median"] = median3x3(out["lr_l"])
+    return out

@@ -1,0 +1,350 @@
+/*
+ * CPU oracle for the SGM stereo path -- TEST INFRASTRUCTURE ONLY.
+ * Only tests/, the smoke check and the benchmark's CPU-baseline leg load it.
+ *
+ * Restates the integer arithmetic of ethan-li-coding/FasterStereoCuda
+ * (paths below are relative to that repository).  Built with
+ * -O3 -fopenmp -ffp-contract=off and no fast-math so that the float steps
+ * (uniqueness test, parabola subpixel, LR average) round exactly as IEEE
+ * single precision does on the GPU.
+ *
+ * Decisions for reference behaviour that is racy or undefined:
+ *   A1  WTA argmin tie -> the smallest d attaining the minimum
+ *   A2  LR check reads the right map as it was before the check
+ *   A3  median is out of place; out-of-image neighbours are INVALID;
+ *       x = 0 and y = 0 pass through unchanged
+ *   A5  WTA rows/columns the reference grid never covers -> INVALID
+ *   A7  LR check runs over xl in [0, W)
+ *   A8  the first pixel of each path is stored (L = C)
+ *   A9  median filters every row
+ *   A10 64-bit offsets
+ *   A12 INVALID_COST=127 boundary terms without +P1, uint8 storage of the
+ *       per-path costs, c3 = D-1 at d* = D-1 (D in {64,128}), +0.5 bias: kept
+ *   A13 right-border census constant 0x00000000FFFFFFFF: kept
+ *   A16 5x5 right-to-left cost skips the last pixel of every D-pixel block
+ *       (cusgm_cc.cu:288); that pixel keeps the left-to-right cost the same
+ *       buffer held: kept
+ *   A17 with the LR check off the right map is INVALID everywhere
+ */
+#include "sgm_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+inline size_t idx2(int x, int y, int W) { return (size_t)y * (size_t)W + (size_t)x; }
+inline size_t idx3(int x, int y, int d, int W, int D) {
+    return ((size_t)y * (size_t)W + (size_t)x) * (size_t)D + (size_t)d;
+}
+
+inline uint32_t popcount64(uint64_t v) { return (uint32_t)__builtin_popcountll(v); }
+
+}  // namespace
+
+extern "C" {
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* Census transform.  cusgm_cc.cu:30-67 (5x5, radius 2x2) and :68-106
+ * (9x7, radius 4x3).  Bits are shifted in MSB-first over the raster order
+ * i = -rh..rh, j = -rw..rw, the centre included; a bit is 1 iff the
+ * neighbour is strictly darker than the centre.  Border pixels take
+ * invalid_val: 0 for the left view, 0xFFFFFFFF for the right view
+ * (cusgm_cc.cu:450-464; for 9x7 the uint32 constant widens to
+ * 0x00000000FFFFFFFF). */
+void orc_census(const uint8_t* img, int W, int H, int cs_type, int is_right,
+                uint64_t* out) {
+    const int rw = cs_type ? 4 : 2;
+    const int rh = cs_type ? 3 : 2;
+    const uint64_t invalid = is_right ? 0xFFFFFFFFull : 0ull;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            uint64_t v = 0;
+            if (x < rw || x >= W - rw || y < rh || y >= H - rh) {
+                v = invalid;
+            } else {
+                const uint8_t c = img[idx2(x, y, W)];
+                for (int i = -rh; i <= rh; ++i)
+                    for (int j = -rw; j <= rw; ++j)
+                        v = (v << 1) | (uint64_t)(img[idx2(x + j, y + i, W)] < c);
+            }
+            out[idx2(x, y, W)] = v;
+        }
+    }
+}
+
+/* Hamming cost volume without initial disparity.  cusgm_cc.cu:118-172 (u64)
+ * and :238-293 (u32), launch geometry :472-523.
+ *   left_to_right:  C[y][x][t] = popc(cb[y][x] ^ cm[y][x - min_d - t])
+ *   right_to_left:  C[y][x][t] = popc(cb[y][x] ^ cm[y][x + min_d + t])
+ * with cb = census of the view being matched and cm = the other view; a
+ * partner column outside [0, W) is replaced by cm[y][0].  Only columns
+ * x >= roi_x = max(0, min_d) are written (cusgm_cc.cu:486-487).
+ * A16: for the 5x5 census the right-to-left kernel's loop stops one pixel
+ * short of each D-pixel block (cusgm_cc.cu:288); such a pixel keeps what
+ * the buffer held, which in StereoCudaImpl::Match is the left-to-right cost
+ * of the same frame (cusgm_ca.cu:4191 reuses the buffer).  Pass that volume
+ * as l2r_cost. */
+void orc_cost(const uint64_t* cs_base, const uint64_t* cs_match, int W, int H,
+              int min_d, int D, int left_to_right, int cs_type,
+              const uint8_t* l2r_cost, uint8_t* cost) {
+    const int roi_x = std::max(0, min_d);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y) {
+        const uint64_t* cb = cs_base + (size_t)y * W;
+        const uint64_t* cm = cs_match + (size_t)y * W;
+        for (int x = roi_x; x < W; ++x) {
+            const bool skipped = (!left_to_right && cs_type == 0 && l2r_cost != nullptr) &&
+                                 (((x - roi_x) % D) == D - 1 || x == W - 1);
+            for (int t = 0; t < D; ++t) {
+                uint8_t c;
+                if (skipped) {
+                    c = l2r_cost[idx3(x, y, t, W, D)];
+                } else {
+                    const int xm = left_to_right ? x - min_d - t : x + min_d + t;
+                    const uint64_t m = (xm >= 0 && xm < W) ? cm[xm] : cm[0];
+                    c = (uint8_t)popcount64(cb[x] ^ m);
+                }
+                cost[idx3(x, y, t, W, D)] = c;
+            }
+        }
+    }
+}
+
+/* One SGM scanline recurrence.  cusgm_ca.cu:1480-1651 (D=64 warp kernels),
+ * :1654-1888 (D=128) and the generic D>=256 kernels (:156-226, :623-685,
+ * :939-1005, :1891-1953) all compute, in uint32,
+ *   L(p0, d) = C(p0, d)
+ *   L(p, d)  = C(p, d) + min(L'(d), B-(d), B+(d), m' + P2) - m'
+ *   B-(d) = d == 0   ? 127 : L'(d-1) + P1
+ *   B+(d) = d == D-1 ? 127 : L'(d+1) + P1
+ *   m = min_d L(p, d)
+ * and store L truncated to uint8.  A8: the first pixel is stored too. */
+static void scan_line(const uint8_t* cost, uint8_t* out, size_t first,
+                      std::ptrdiff_t step, int n, int D, uint32_t p1, uint32_t p2,
+                      std::vector<uint32_t>& Lp, std::vector<uint32_t>& Ln) {
+    if (n <= 0) return;
+    size_t off = first;
+    uint32_t m = 0xFFFFFFFFu;
+    for (int d = 0; d < D; ++d) {
+        Lp[d] = cost[off + d];
+        out[off + d] = (uint8_t)Lp[d];
+        m = std::min(m, Lp[d]);
+    }
+    for (int i = 1; i < n; ++i) {
+        off = (size_t)((std::ptrdiff_t)off + step);
+        uint32_t mn = 0xFFFFFFFFu;
+        for (int d = 0; d < D; ++d) {
+            const uint32_t bm = (d == 0) ? ORC_INVALID_COST : Lp[d - 1] + p1;
+            const uint32_t bp = (d == D - 1) ? ORC_INVALID_COST : Lp[d + 1] + p1;
+            uint32_t v = std::min(std::min(std::min(Lp[d], bm), bp), m + p2);
+            const uint32_t l = (uint32_t)cost[off + d] + (v - m);
+            Ln[d] = l;
+            out[off + d] = (uint8_t)l;
+            mn = std::min(mn, l);
+        }
+        m = mn;
+        std::swap(Lp, Ln);
+    }
+}
+
+/* Aggregation along one direction over the region x in [roi_x, W),
+ * y in [0, H) (cusgm_ca.cu:3991-4000, 4098-4108). */
+void orc_aggregate(const uint8_t* cost, int W, int H, int D, int roi_x,
+                   int dir, int p1, int p2, uint8_t* out) {
+    const std::ptrdiff_t px = (std::ptrdiff_t)D;
+    const std::ptrdiff_t row = (std::ptrdiff_t)W * D;
+    const int roi_w = W - roi_x;
+    if (dir == ORC_DIR_LR || dir == ORC_DIR_RL) {
+#pragma omp parallel
+        {
+            std::vector<uint32_t> a(D), b(D);
+#pragma omp for schedule(static)
+            for (int y = 0; y < H; ++y) {
+                if (dir == ORC_DIR_LR)
+                    scan_line(cost, out, idx3(roi_x, y, 0, W, D), px, roi_w, D, p1, p2, a, b);
+                else
+                    scan_line(cost, out, idx3(W - 1, y, 0, W, D), -px, roi_w, D, p1, p2, a, b);
+            }
+        }
+    } else {
+#pragma omp parallel
+        {
+            std::vector<uint32_t> a(D), b(D);
+#pragma omp for schedule(static)
+            for (int x = roi_x; x < W; ++x) {
+                if (dir == ORC_DIR_UD)
+                    scan_line(cost, out, idx3(x, 0, 0, W, D), row, H, D, p1, p2, a, b);
+                else
+                    scan_line(cost, out, idx3(x, H - 1, 0, W, D), -row, H, D, p1, p2, a, b);
+            }
+        }
+    }
+}
+
+/* Winner-takes-all with uniqueness and parabola subpixel.
+ * cusgm_ca.cu:3606-3699 (D=64), :3701-3809 (D=128), :3149-3232 (generic),
+ * SubPixBias :150-154.
+ *   S(d) = sum of the four stored uint8 path costs
+ *   border x < 5 || x > W-5 || y < 5 || y > H-5 -> INVALID
+ *   c2 = min S, d* = argmin (A1: smallest d)
+ *   D in {64,128}: INVALID iff d* <= 0; at d* = D-1 the reference reads c3
+ *     from the shared slot that holds the argmin, i.e. c3 = (float)(D-1)
+ *   other D: INVALID iff d* <= 0 || d* >= D-1
+ *   uniq = min(c1 - c2, c3 - c2); INVALID iff uniq <= c2 * (1 - uniqueness)
+ *   disp = (d* + min_d) + (c1 - c3 + den) / (2 den),
+ *          den = max(1, c1 + c3 - 2 c2)
+ * A5: the D=64/128 kernels cover rows [0, 2*(H/2)) and the generic kernel
+ * all rows; columns below roi_x are never written.  Uncovered -> INVALID. */
+void orc_wta(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2,
+             const uint8_t* p3, int W, int H, int D, int min_d, int roi_x,
+             float uniqueness, float* disp) {
+    const bool warp_rule = (D == 64 || D == 128);
+    const int rows_covered = warp_rule ? 2 * (H / 2) : H;
+    const float unique_threshold = 1.0f - uniqueness;
+#pragma omp parallel
+    {
+        std::vector<uint32_t> S(D);
+#pragma omp for schedule(static)
+        for (int y = 0; y < H; ++y) {
+            for (int x = 0; x < W; ++x) {
+                float& out = disp[idx2(x, y, W)];
+                out = ORC_INVALID_DISP;
+                if (x < roi_x || y >= rows_covered) continue;
+                if (x > W - 5 || y > H - 5 || x < 5 || y < 5) continue;
+                const size_t base = idx3(x, y, 0, W, D);
+                uint32_t c2 = 0xFFFFFFFFu;
+                int dstar = 0;
+                for (int d = 0; d < D; ++d) {
+                    S[d] = (uint32_t)p0[base + d] + p1[base + d] + p2[base + d] + p3[base + d];
+                    if (S[d] < c2) { c2 = S[d]; dstar = d; }
+                }
+                if (dstar <= 0) continue;
+                if (!warp_rule && dstar >= D - 1) continue;
+                const float c1 = (float)S[dstar - 1];
+                const float c2f = (float)c2;
+                const float c3 = (dstar == D - 1) ? (float)(D - 1) : (float)S[dstar + 1];
+                const float a = c1 - c2f, b = c3 - c2f;
+                const float uniq = (a < b) ? a : b;
+                if (uniq <= c2f * unique_threshold) continue;
+                const float t = c1 + c3 - 2.0f * c2f;
+                const float den = (1.0f < t) ? t : 1.0f;
+                const float bias = (c1 - c3 + den) / (2.0f * den);
+                out = (float)(dstar + min_d) + bias;
+            }
+        }
+    }
+}
+
+/* Left-right consistency check.  cusgm_ca.cu:3873-3898.
+ *   xr = round-half-away(xl - dL) (the +-0.5 is a double add)
+ *   xr outside [0, W)      -> L INVALID
+ *   |dL - dR[xr]| > thr    -> L INVALID and R[xr] INVALID
+ *   otherwise              -> L = (dL + dR[xr]) / 2
+ * A2: dR is read from disp_r_in (the map before the check); invalidations
+ * of the right map go to disp_r_out (may be NULL).  A7: xl in [0, W). */
+void orc_lrcheck(float* disp_l, const float* disp_r_in, float* disp_r_out,
+                 int W, int H, float threshold) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y) {
+        float* left = disp_l + (size_t)y * W;
+        const float* right = disp_r_in + (size_t)y * W;
+        for (int xl = 0; xl < W; ++xl) {
+            const float dl = left[xl];
+            if (dl == ORC_INVALID_DISP) continue;
+            const float xrf = (float)xl - dl;
+            const int xr = (xrf > 0) ? (int)((double)xrf + 0.5) : (int)((double)xrf - 0.5);
+            if (xr < 0 || xr >= W) { left[xl] = ORC_INVALID_DISP; continue; }
+            const float dr = right[xr];
+            if (std::fabs(dl - dr) > threshold) {
+                left[xl] = ORC_INVALID_DISP;
+                if (disp_r_out) disp_r_out[(size_t)y * W + xr] = ORC_INVALID_DISP;
+            } else {
+                left[xl] = (dl + dr) / 2.0f;
+            }
+        }
+    }
+}
+
+/* 3x3 median.  cusgm_df.cu:16-75 (sort network; the value it returns is the
+ * exact median of the nine inputs), launch :346-355.  A3/A9: computed out of
+ * place over every row; x = 0 and y = 0 copy the input; neighbours outside
+ * the image count as INVALID.  The median is taken here by sorting, which is
+ * independent of the network the GPU uses. */
+void orc_median3x3(const float* in, float* out, int W, int H) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            if (x == 0 || y == 0) { out[idx2(x, y, W)] = in[idx2(x, y, W)]; continue; }
+            float v[9];
+            int k = 0;
+            for (int i = -1; i <= 1; ++i)
+                for (int j = -1; j <= 1; ++j) {
+                    const int xx = x + j, yy = y + i;
+                    v[k++] = (xx < W && yy < H) ? in[idx2(xx, yy, W)] : ORC_INVALID_DISP;
+                }
+            std::sort(v, v + 9);
+            out[idx2(x, y, W)] = v[4];
+        }
+    }
+}
+
+/* Plain single-level pipeline in StereoCudaImpl::Match order
+ * (StereoCudaImpl.cpp:507-546): census L, census R, cost L->R, 4 paths,
+ * WTA L, [cost R->L, 4 paths, WTA R, LR check], median. */
+int orc_match(const uint8_t* left, const uint8_t* right, int W, int H,
+              int min_d, int D, const orc_options* opt, float* disp_l,
+              float* disp_r) {
+    if (!left || !right || !disp_l || !opt || W <= 0 || H <= 0 || D <= 0) return -1;
+    const size_t npix = (size_t)W * H;
+    const size_t ncell = npix * (size_t)D;
+    const int roi_x = std::max(0, min_d);
+    std::vector<uint64_t> csl(npix), csr(npix);
+    std::vector<uint8_t> cost(ncell), l2r;
+    std::vector<uint8_t> path[4];
+    for (auto& p : path) p.assign(ncell, 0);
+    std::vector<float> dl(npix), dr(npix, ORC_INVALID_DISP);
+
+    orc_census(left, W, H, opt->cs_type, 0, csl.data());
+    orc_census(right, W, H, opt->cs_type, 1, csr.data());
+    orc_cost(csl.data(), csr.data(), W, H, min_d, D, 1, opt->cs_type, nullptr, cost.data());
+    for (int r = 0; r < 4; ++r)
+        orc_aggregate(cost.data(), W, H, D, roi_x, r, opt->p1, opt->p2, path[r].data());
+    orc_wta(path[0].data(), path[1].data(), path[2].data(), path[3].data(), W, H, D, min_d,
+            roi_x, opt->uniqueness, dl.data());
+    if (opt->do_lr_check) {
+        if (opt->cs_type == 0) l2r = cost;
+        orc_cost(csr.data(), csl.data(), W, H, min_d, D, 0, opt->cs_type,
+                 opt->cs_type == 0 ? l2r.data() : nullptr, cost.data());
+        for (int r = 0; r < 4; ++r)
+            orc_aggregate(cost.data(), W, H, D, roi_x, r, opt->p1, opt->p2, path[r].data());
+        std::vector<float> dr_in(npix);
+        orc_wta(path[0].data(), path[1].data(), path[2].data(), path[3].data(), W, H, D, min_d,
+                roi_x, opt->uniqueness, dr_in.data());
+        dr = dr_in;
+        orc_lrcheck(dl.data(), dr_in.data(), dr.data(), W, H, opt->lr_threshold);
+    }
+    if (opt->do_median) {
+        orc_median3x3(dl.data(), disp_l, W, H);
+    } else {
+        std::memcpy(disp_l, dl.data(), npix * sizeof(float));
+    }
+    if (disp_r) std::memcpy(disp_r, dr.data(), npix * sizeof(float));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+/*
+ * CPU oracle for the SGM stereo path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This is a plain C++/OpenMP restatement of the reference kernels' integer
+ * arithmetic (ethan-li-coding/FasterStereoCuda, SgmStereoCuda/cusgm_*.cu).
+ * It is the checker for the HIP path and the CPU baseline in the benchmark.
+ * Nothing in the product library links or calls it.
+ *
+ * Every function names the reference lines it follows.  Where the reference
+ * behaviour is racy or undefined, the oracle implements one documented
+ * decision (ids A1..A17), and the HIP kernels implement the same decision.
+ *
+ * Layouts (all dense, row-major, no pitch):
+ *   image    u8  [H][W]
+ *   census   u64 [H][W]     (5x5 census uses the low 25 bits)
+ *   cost     u8  [H][W][D]  (d fastest)
+ *   path     u8  [H][W][D]  one volume per aggregation direction
+ *   disp     f32 [H][W]     invalid = -999999.0f
+ */
+#ifndef SGM_ORACLE_H_
+#define SGM_ORACLE_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_INVALID_DISP (-999999.0f)
+#define ORC_INVALID_COST 127u
+
+/* aggregation directions, in the order the reference stores them
+ * (cusgm_ca.cu:4101-4108): 0 top->bottom, 1 left->right, 2 bottom->top,
+ * 3 right->left */
+enum { ORC_DIR_UD = 0, ORC_DIR_LR = 1, ORC_DIR_DU = 2, ORC_DIR_RL = 3 };
+
+typedef struct orc_options {
+    int32_t cs_type;          /* 0 = 5x5, 1 = 9x7 */
+    int32_t p1;
+    int32_t p2;
+    float   uniqueness;       /* CuSGMOption::unique_threshold */
+    int32_t do_lr_check;
+    float   lr_threshold;
+    int32_t do_median;
+} orc_options;
+
+int  orc_num_threads(void);
+
+void orc_census(const uint8_t* img, int W, int H, int cs_type, int is_right,
+                uint64_t* out);
+
+void orc_cost(const uint64_t* cs_base, const uint64_t* cs_match, int W, int H,
+              int min_d, int D, int left_to_right, int cs_type,
+              const uint8_t* l2r_cost, uint8_t* cost);
+
+void orc_aggregate(const uint8_t* cost, int W, int H, int D, int roi_x,
+                   int dir, int p1, int p2, uint8_t* out);
+
+void orc_wta(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2,
+             const uint8_t* p3, int W, int H, int D, int min_d, int roi_x,
+             float uniqueness, float* disp);
+
+void orc_lrcheck(float* disp_l, const float* disp_r_in, float* disp_r_out,
+                 int W, int H, float threshold);
+
+void orc_median3x3(const float* in, float* out, int W, int H);
+
+/* Whole plain pipeline (StereoCudaImpl::Match order).  disp_r may be NULL.
+ * Returns 0 on success. */
+int orc_match(const uint8_t* left, const uint8_t* right, int W, int H,
+              int min_d, int D, const orc_options* opt, float* disp_l,
+              float* disp_r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

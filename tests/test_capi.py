@@ -1,0 +1,126 @@
+"""The C ABI boundary, checked on the CPU (no compute calls):
+  - libfsgm.so loads and exports every function include/sgm_capi.h declares
+  - libStereoCuda.so exports every StereoCuda member include/StereoCuda.h declares
+  - the ctypes mirror of sgm_option has the header's field order and types
+  - argument validation happens before the device is touched
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from fasterstereocuda_amd import _lib
+from fasterstereocuda_amd.stereo import CuSGMOption, StereoCuda
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+PKG = os.path.join(ROOT, "fasterstereocuda_amd")
+
+
+def _header_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sgm_\w+)\s*\(", text)))
+
+
+def _exports(lib, demangle=False):
+    out = subprocess.run(["nm", "-D", "--defined-only"] + (["-C"] if demangle else []) + [lib],
+                         capture_output=True, text=True, check=True).stdout
+    return out
+
+
+def test_capi_exports_every_declared_function():
+    declared = [f for f in _header_functions(os.path.join(INCLUDE, "sgm_capi.h"))]
+    assert len(declared) >= 20
+    syms = set(line.split()[-1] for line in _exports(os.path.join(PKG, "libfsgm.so")).splitlines())
+    missing = [f for f in declared if f not in syms]
+    assert not missing, f"declared but not exported: {missing}"
+    lib = _lib.lib()
+    for f in declared:
+        assert hasattr(lib, f)
+    assert set(declared) == set(_lib.SIGNATURES), "ctypes signature table out of date"
+
+
+def test_stereocuda_class_exports():
+    text = open(os.path.join(INCLUDE, "StereoCuda.h")).read()
+    members = set(re.findall(r"\b(Init2?|Match2?|SetMinDisparity|GetDispartyRange|Release|LastError|"
+                             r"GetRoiFromDispMap|get_invad_float|get_invad_short|get_level_range|"
+                             r"MallocPageLockedPtr|FreePageLockedPtr)\s*\(", text))
+    exported = _exports(os.path.join(PKG, "libStereoCuda.so"), demangle=True)
+    for m in sorted(members):
+        assert f"StereoCuda::{m}(" in exported, f"StereoCuda::{m} not exported"
+    assert "StereoCuda::StereoCuda()" in exported and "StereoCuda::~StereoCuda()" in exported
+
+
+def test_option_struct_matches_header():
+    text = open(os.path.join(INCLUDE, "sgm_capi.h")).read()
+    body = re.search(r"typedef struct sgm_option \{(.*?)\} sgm_option;", text, re.S).group(1)
+    fields = re.findall(r"(int32_t|float)\s+(\w+);", body)
+    py = [(n, t) for n, t in _lib.SgmOption._fields_]
+    assert [n for _, n in fields] == [n for n, _ in py]
+    for (ct, _), (_, pt) in zip(fields, py):
+        assert (ct == "float") == (pt is ctypes.c_float)
+    # and the CuSGMOption mirror fills every field in order
+    o = CuSGMOption.benchmark()
+    c = o.to_c()
+    assert (c.p1, c.p2_init, c.cs_type, c.do_lr_check, c.do_median_filter) == (10, 150, 1, 1, 1)
+    assert abs(c.unique_threshold - 0.95) < 1e-7 and c.lr_threshold == 1.0
+
+
+@pytest.mark.parametrize("args,code,needle", [
+    ((0, 10, 0, 64), _lib.SGM_E_INVALID_ARG, "size"),
+    ((100, 10, 0, 96), _lib.SGM_E_INVALID_ARG, "disp_range"),
+    ((100, 10, 0, 1024), _lib.SGM_E_INVALID_ARG, "disp_range"),
+    ((100, 10, 100, 64), _lib.SGM_E_INVALID_ARG, "min_disparity"),
+])
+def test_init_validates_before_device(args, code, needle):
+    lib = _lib.lib()
+    h = lib.sgm_create()
+    opt = CuSGMOption.benchmark().to_c()
+    rc = lib.sgm_init(h, *args, ctypes.byref(opt), 0)
+    assert rc == code
+    assert needle in _lib.last_error(h)
+    lib.sgm_destroy(h)
+
+
+@pytest.mark.parametrize("field,value,code", [
+    ("do_remove_peaks", True, _lib.SGM_E_UNSUPPORTED),
+    ("post_filter_type", 1, _lib.SGM_E_UNSUPPORTED),
+    ("morphology_type", 2, _lib.SGM_E_UNSUPPORTED),
+    ("p1", -1, _lib.SGM_E_INVALID_ARG),
+    ("p2_init", 70000, _lib.SGM_E_INVALID_ARG),
+    ("cs_type", 3, _lib.SGM_E_INVALID_ARG),
+])
+def test_option_validation(field, value, code):
+    lib = _lib.lib()
+    h = lib.sgm_create()
+    opt = CuSGMOption.benchmark(**{field: value}).to_c()
+    assert lib.sgm_init(h, 100, 50, 0, 64, ctypes.byref(opt), 0) == code
+    lib.sgm_destroy(h)
+
+
+def test_uninitialised_handle_and_constants():
+    st = StereoCuda()
+    import numpy as np
+    a = np.zeros((4, 4), np.uint8)
+    o = np.zeros((4, 4), np.float32)
+    st.width, st.height = 4, 4
+    assert not st.Match(a, a, o)
+    assert "not initialised" in st.last_error
+    st.Release()
+    st.Release()
+    assert StereoCuda.get_invad_float() == -999999.0
+    assert StereoCuda.get_invad_short() == -32768
+    assert StereoCuda.get_level_range() == 32
+
+
+def test_roi_from_disparity_host_scan():
+    import numpy as np
+    d = np.full((10, 12), -999999.0, np.float32)
+    d[3, 4] = 1.0
+    d[7, 9] = 2.0
+    d[5, 2] = 3.0
+    roi = StereoCuda.GetRoiFromDispMap(d)
+    assert (roi.x, roi.y, roi.w, roi.h) == (2, 3, 8, 5)

@@ -1,0 +1,168 @@
+"""Parity of the HIP plain path with the CPU oracle, stage by stage and end
+to end.  Integer stages (census, cost, per-path aggregation) must match bit
+for bit; the float disparity maps must match bit for bit as well (the HIP
+kernels reproduce the reference's float steps exactly), which is stricter
+than the 1e-5 px bound on subpixel output."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from fasterstereocuda_amd import scenes, stages
+from fasterstereocuda_amd.stereo import CuSGMOption, StereoCuda
+
+pytestmark = pytest.mark.gpu
+
+# (W, H, D, min_d, cs_type, p1, p2): the 450x375 workload, ragged sizes,
+# every supported D, the 5x5 census, positive / negative min_d, P1 > 127.
+CASES = [
+    (450, 375, 64, 0, 1, 10, 150),
+    (97, 61, 64, 0, 1, 10, 150),
+    (130, 45, 128, 0, 1, 10, 150),
+    (300, 70, 256, 0, 1, 10, 150),
+    (121, 50, 64, 0, 0, 8, 32),
+    (200, 33, 64, 7, 1, 10, 150),
+    (150, 40, 64, -5, 1, 10, 150),
+    (90, 30, 64, 0, 1, 200, 300),
+    (600, 21, 512, 0, 1, 10, 150),
+    (77, 9, 128, 3, 0, 10, 150),
+]
+UNIQ, LR_THR = 0.95, 1.0
+
+
+def _ids(c):
+    return "W{}H{}D{}m{}cs{}p{}-{}".format(*c)
+
+
+def _pair(W, H, D, seed=0):
+    p = scenes.cone_like(seed, H=H, W=W, D=D)
+    return p.left, p.right
+
+
+def _eq_float(a: np.ndarray, b: np.ndarray, what: str):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    assert a.shape == b.shape, what
+    same_bits = a.view(np.uint32) == b.view(np.uint32)
+    if not same_bits.all():
+        bad = np.argwhere(~same_bits)[:5]
+        pytest.fail(f"{what}: {int((~same_bits).sum())} of {a.size} differ, e.g. "
+                    + ", ".join(f"{tuple(i)}: gpu {a[tuple(i)]!r} oracle {b[tuple(i)]!r}" for i in bad))
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_stages_match_oracle(cuda, case):
+    W, H, D, min_d, cs, p1, p2 = case
+    print(f"[stages] {case}", flush=True)
+    left, right = _pair(W, H, D)
+    ref = oracle.stages(left, right, min_d, D, cs_type=cs, p1=p1, p2=p2, uniqueness=UNIQ,
+                        lr_threshold=LR_THR)
+    roi_x = max(0, min_d)
+    L = torch.from_numpy(left).to(cuda)
+    R = torch.from_numpy(right).to(cuda)
+    cl = stages.census(L, cs, False)
+    cr = stages.census(R, cs, True)
+    np.testing.assert_array_equal(cl.cpu().numpy().view(np.uint64), ref["census_l"])
+    np.testing.assert_array_equal(cr.cpu().numpy().view(np.uint64), ref["census_r"])
+
+    for view, (cb, cm, l2r) in (("l", (cl, cr, True)), ("r", (cr, cl, False))):
+        c = stages.cost(cb, cm, min_d, D, l2r, cs)
+        np.testing.assert_array_equal(c.cpu().numpy()[:, roi_x:], ref[f"cost_{view}"][:, roi_x:],
+                                      err_msg=f"cost {view}")
+        paths = stages.aggregate(c, min_d, p1, p2)
+        for r in range(4):
+            np.testing.assert_array_equal(paths[r].cpu().numpy()[:, roi_x:],
+                                          ref[f"paths_{view}"][r][:, roi_x:],
+                                          err_msg=f"path {stages.DIRS[r]} view {view}")
+        d = stages.wta(paths, min_d, UNIQ)
+        _eq_float(d.cpu().numpy(), ref[f"wta_{view}"], f"wta {view}")
+
+    wl = torch.from_numpy(ref["wta_l"]).to(cuda)
+    wr = torch.from_numpy(ref["wta_r"]).to(cuda)
+    out_l, out_r = stages.post(wl, wr, True, LR_THR, True, want_right=True)
+    _eq_float(out_l.cpu().numpy(), ref["median"], "LR + median")
+    _eq_float(out_r.cpu().numpy(), ref["lr_r"], "right map after LR")
+    lr_only, _ = stages.post(wl, wr, True, LR_THR, False)
+    _eq_float(lr_only.cpu().numpy(), ref["lr_l"], "LR only")
+    med_only, _ = stages.post(wl, None, False, LR_THR, True)
+    _eq_float(med_only.cpu().numpy(), oracle.median3x3(ref["wta_l"]), "median only")
+
+
+@pytest.mark.parametrize("case", CASES[:6], ids=_ids)
+@pytest.mark.parametrize("lr,median", [(True, True), (False, True), (True, False)])
+def test_stereocuda_match_matches_oracle(cuda, case, lr, median):
+    W, H, D, min_d, cs, p1, p2 = case
+    print(f"[match] {case} lr={lr} median={median}", flush=True)
+    left, right = _pair(W, H, D, seed=3)
+    exp_l, exp_r = oracle.match(left, right, min_d, D, cs_type=cs, p1=p1, p2=p2, uniqueness=UNIQ,
+                                do_lr_check=lr, lr_threshold=LR_THR, do_median=median)
+    st = StereoCuda()
+    opt = CuSGMOption.benchmark(cs_type=cs, p1=p1, p2_init=p2, do_lr_check=lr,
+                                do_median_filter=median)
+    assert st.Init(W, H, min_d, D, opt), st.last_error
+    dl = np.empty((H, W), np.float32)
+    dr = np.empty((H, W), np.float32)
+    for _ in range(2):   # run twice: results must not depend on buffer history
+        assert st.Match(left, right, dl, dr), st.last_error
+        _eq_float(dl, exp_l, "left map")
+        _eq_float(dr, exp_r, "right map")
+    st.Release()
+
+
+def test_driving_full_size(cuda):
+    pair = scenes.driving(0)
+    print("[match] driving 1242x375", flush=True)
+    exp_l, exp_r = oracle.match(pair.left, pair.right, 0, 64)
+    st = StereoCuda()
+    assert st.Init(1242, 375, 0, 64, CuSGMOption.benchmark()), st.last_error
+    dl = np.empty((375, 1242), np.float32)
+    dr = np.empty((375, 1242), np.float32)
+    assert st.Match(pair.left, pair.right, dl, dr), st.last_error
+    _eq_float(dl, exp_l, "left map")
+    _eq_float(dr, exp_r, "right map")
+    # device-resident entry point gives the same bytes
+    Ld = torch.from_numpy(pair.left).to(cuda)
+    Rd = torch.from_numpy(pair.right).to(cuda)
+    od = torch.empty((375, 1242), dtype=torch.float32, device=cuda)
+    assert st.match_device(Ld, Rd, od)
+    torch.cuda.synchronize()
+    _eq_float(od.cpu().numpy(), exp_l, "device entry point")
+
+
+def test_cpp_dropin_demo(cuda, tmp_path):
+    """The C++ StereoCuda class (libStereoCuda.so) used the way a reference
+    caller uses it, through the stereo_demo program."""
+    demo = os.path.join(os.path.dirname(oracle.__file__), "..", "fasterstereocuda_amd", "stereo_demo")
+    pair = scenes.cone_like(7)
+    H, W = pair.left.shape
+    lp, rp = tmp_path / "l.u8", tmp_path / "r.u8"
+    ol, orr = tmp_path / "dl.f32", tmp_path / "dr.f32"
+    pair.left.tofile(lp)
+    pair.right.tofile(rp)
+    print("[demo] cone 450x375", flush=True)
+    res = subprocess.run([demo, str(lp), str(rp), str(W), str(H), "0", "64", str(ol), str(orr)],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr + res.stdout
+    exp_l, exp_r = oracle.match(pair.left, pair.right, 0, 64)
+    _eq_float(np.fromfile(ol, np.float32).reshape(H, W), exp_l, "demo left")
+    _eq_float(np.fromfile(orr, np.float32).reshape(H, W), exp_r, "demo right")
+
+
+def test_errors_are_reported(cuda):
+    st = StereoCuda()
+    assert not st.Init(100, 50, 0, 96, CuSGMOption.benchmark())
+    assert "disp_range" in st.last_error
+    assert not st.Init(100, 50, 0, 64, CuSGMOption.benchmark(do_remove_peaks=True))
+    assert "remove_peaks" in st.last_error
+    left = np.zeros((50, 100), np.uint8)
+    out = np.empty((50, 100), np.float32)
+    assert not st.Match(left, left, out)          # not initialised
+    assert st.Init(100, 50, 0, 64, CuSGMOption.benchmark())
+    assert st.Match(left, left, out)
+    assert st.GetDispartyRange() == 64
+    st.Release()
+    st.Release()                                   # double release is safe

@@ -1,0 +1,322 @@
+"""Pins the CPU oracle (oracle/sgm_oracle.cpp) without trusting its own code.
+
+The reference ships no tests, golden vectors or ground truth, so each
+oracle function is checked against an independent statement of the same
+arithmetic, against closed forms, or against a library routine:
+  census     positional bit formula on tiny images; constant image; borders
+  cost       explicit popcount of the partner census; shifted-image zero cost
+  aggregate  the recurrence written as a min over all d' with a penalty
+             matrix (structurally different from the oracle's 4-term min);
+             P1 = P2 = 0 makes every path equal to the cost
+  wta        numpy argmin (first occurrence) + float32 formulas; hand cases
+  lrcheck    numpy restatement + hand cases
+  median     scipy.ndimage.median_filter
+  pipeline   synthetic scenes with known disparity (error rates)
+"""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import oracle
+from fasterstereocuda_amd import scenes
+
+INV = np.float32(-999999.0)
+
+
+def census_positional(img, cs_type, is_right):
+    rw, rh = (4, 3) if cs_type else (2, 2)
+    H, W = img.shape
+    nbits = (2 * rw + 1) * (2 * rh + 1)
+    out = np.zeros((H, W), np.uint64)
+    for y in range(H):
+        for x in range(W):
+            if x < rw or x >= W - rw or y < rh or y >= H - rh:
+                out[y, x] = 0xFFFFFFFF if is_right else 0
+                continue
+            v = 0
+            k = 0
+            for i in range(-rh, rh + 1):
+                for j in range(-rw, rw + 1):
+                    if img[y + i, x + j] < img[y, x]:
+                        v |= 1 << (nbits - 1 - k)
+                    k += 1
+            out[y, x] = v
+    return out
+
+
+@pytest.mark.parametrize("cs_type", [0, 1])
+@pytest.mark.parametrize("is_right", [False, True])
+def test_census_positional(cs_type, is_right):
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, size=(12, 16), dtype=np.uint8)
+    img[4, 5] = img[4, 6]  # a tie: equal is not darker
+    np.testing.assert_array_equal(oracle.census(img, cs_type, is_right),
+                                  census_positional(img, cs_type, is_right))
+
+
+def test_census_closed_forms():
+    flat = np.full((20, 30), 77, np.uint8)
+    c = oracle.census(flat, 1, False)
+    assert (c[3:-3, 4:-4] == 0).all()
+    cr = oracle.census(flat, 1, True)
+    assert (cr[:3] == 0xFFFFFFFF).all() and (cr[:, :4] == 0xFFFFFFFF).all()
+    # an isolated bright pixel: its census has every bit but the centre set
+    img = np.zeros((20, 30), np.uint8)
+    img[10, 10] = 200
+    v = int(oracle.census(img, 1, False)[10, 10])
+    assert v == ((1 << 63) - 1) & ~(1 << 31)
+    v5 = int(oracle.census(img, 0, False)[10, 10])
+    assert v5 == ((1 << 25) - 1) & ~(1 << 12)
+
+
+@pytest.mark.parametrize("l2r", [True, False])
+@pytest.mark.parametrize("min_d", [0, 3, -4])
+def test_cost_explicit(l2r, min_d):
+    rng = np.random.default_rng(11)
+    H, W, D = 6, 40, 64
+    cb = rng.integers(0, 2**63, size=(H, W), dtype=np.uint64)
+    cm = rng.integers(0, 2**63, size=(H, W), dtype=np.uint64)
+    got = oracle.cost(cb, cm, min_d, D, l2r, 1)
+    roi_x = max(0, min_d)
+    for y in range(H):
+        for x in range(roi_x, W):
+            for t in range(0, D, 7):
+                xm = x - min_d - t if l2r else x + min_d + t
+                m = cm[y, xm] if 0 <= xm < W else cm[y, 0]
+                assert got[y, x, t] == bin(int(cb[y, x]) ^ int(m)).count("1")
+
+
+def test_cost_of_shifted_image_is_zero():
+    rng = np.random.default_rng(2)
+    base = rng.integers(0, 256, size=(30, 120), dtype=np.uint8)
+    delta = 9
+    left = base[:, delta:]              # left[x] = base[x + delta]
+    right = base[:, :-delta]            # right[x] = base[x]  -> left[x] = right[x + delta]
+    # so the left pixel x sees itself at right x + delta: use min_d = -delta
+    cl = oracle.census(left, 1, False)
+    cr = oracle.census(right, 1, True)
+    c = oracle.cost(cl, cr, -delta, 64, True, 1)
+    assert (c[3:-3, 4:-4 - delta, 0] == 0).all()
+
+
+def test_cost_5x5_right_view_keeps_left_cost_in_skipped_pixels():
+    rng = np.random.default_rng(4)
+    H, W, D = 5, 150, 64
+    cl = rng.integers(0, 2**25, size=(H, W), dtype=np.uint64)
+    cr = rng.integers(0, 2**25, size=(H, W), dtype=np.uint64)
+    c_l = oracle.cost(cl, cr, 0, D, True, 0)
+    c_r = oracle.cost(cr, cl, 0, D, False, 0, l2r_cost=c_l, out=c_l.copy())
+    skipped = [x for x in range(W) if x % D == D - 1 or x == W - 1]
+    for x in range(W):
+        if x in skipped:
+            np.testing.assert_array_equal(c_r[:, x], c_l[:, x])
+        else:
+            for t in (0, 5, 63):
+                xm = x + t
+                m = cl[:, xm] if xm < W else cl[:, 0]
+                exp = [bin(int(a) ^ int(b)).count("1") for a, b in zip(cr[:, x], m)]
+                np.testing.assert_array_equal(c_r[:, x, t], exp)
+
+
+def aggregate_penalty_matrix(cost, roi_x, direction, p1, p2):
+    """L(p,d) = C(p,d) + min(min_d' L'(d') + pen(d,d'), 127 if d at an end) - m',
+    pen = 0 / P1 / P2 for |d-d'| = 0 / 1 / >1; stored mod 256."""
+    H, W, D = cost.shape
+    d = np.arange(D)
+    pen = np.where(d[:, None] == d[None, :], 0, np.where(abs(d[:, None] - d[None, :]) == 1, p1, p2))
+    pen = pen.astype(np.int64)
+    ends = np.full(D, np.iinfo(np.int64).max)
+    ends[0] = ends[-1] = 127
+    out = np.zeros_like(cost)
+    if direction in (1, 3):
+        lines = [[(y, x) for x in range(roi_x, W)] for y in range(H)]
+    else:
+        lines = [[(y, x) for y in range(H)] for x in range(roi_x, W)]
+    for line in lines:
+        if direction in (2, 3):
+            line = line[::-1]
+        prev = None
+        for (y, x) in line:
+            c = cost[y, x].astype(np.int64)
+            if prev is None:
+                cur = c
+            else:
+                m = prev.min()
+                best = (prev[None, :] + pen).min(axis=1)
+                cur = c + np.minimum(best, ends) - m
+            out[y, x] = (cur % 256).astype(np.uint8)
+            prev = cur
+    return out
+
+
+@pytest.mark.parametrize("direction", [0, 1, 2, 3])
+@pytest.mark.parametrize("p1,p2,roi_x", [(10, 150, 0), (8, 32, 2), (200, 300, 0), (3, 250, 1)])
+def test_aggregate_penalty_matrix(direction, p1, p2, roi_x):
+    rng = np.random.default_rng(direction * 10 + p1)
+    cost = rng.integers(0, 64, size=(7, 9, 64), dtype=np.uint8)
+    cost[:, :, 0] = rng.integers(0, 3, size=(7, 9))      # cheap ends make the 127 term bind
+    got = oracle.aggregate(cost, roi_x, direction, p1, p2)
+    exp = aggregate_penalty_matrix(cost, roi_x, direction, p1, p2)
+    np.testing.assert_array_equal(got[:, roi_x:], exp[:, roi_x:])
+
+
+def test_aggregate_zero_penalties_is_identity():
+    rng = np.random.default_rng(1)
+    cost = rng.integers(0, 64, size=(10, 12, 128), dtype=np.uint8)
+    for r in range(4):
+        np.testing.assert_array_equal(oracle.aggregate(cost, 0, r, 0, 0), cost)
+
+
+def wta_numpy(paths, min_d, roi_x, uniq):
+    H, W, D = paths[0].shape
+    S = sum(p.astype(np.int64) for p in paths)
+    out = np.full((H, W), INV, np.float32)
+    warp = D in (64, 128)
+    rows = 2 * (H // 2) if warp else H
+    ut = np.float32(1.0) - np.float32(uniq)
+    for y in range(H):
+        for x in range(W):
+            if x < roi_x or y >= rows or x > W - 5 or y > H - 5 or x < 5 or y < 5:
+                continue
+            s = S[y, x]
+            ds = int(np.argmin(s))
+            if ds <= 0 or (not warp and ds >= D - 1):
+                continue
+            c1 = np.float32(s[ds - 1])
+            c2 = np.float32(s[ds])
+            c3 = np.float32(D - 1) if ds == D - 1 else np.float32(s[ds + 1])
+            u = min(c1 - c2, c3 - c2)
+            if u <= c2 * ut:
+                continue
+            den = max(np.float32(1.0), c1 + c3 - np.float32(2.0) * c2)
+            out[y, x] = np.float32(ds + min_d) + (c1 - c3 + den) / (np.float32(2.0) * den)
+    return out
+
+
+@pytest.mark.parametrize("D,min_d,roi_x", [(64, 0, 0), (128, 4, 4), (256, -3, 0)])
+def test_wta_numpy(D, min_d, roi_x):
+    rng = np.random.default_rng(D)
+    H, W = 13, 17
+    paths = [rng.integers(0, 40, size=(H, W, D), dtype=np.uint8) for _ in range(4)]
+    paths[0][6, 7, :] = 200
+    paths[0][6, 7, D - 1] = 0             # argmin at D-1
+    paths[0][7, 7, :] = 200
+    paths[0][7, 7, 3] = 0
+    paths[0][7, 7, 9] = 0                 # a tie: the smaller d wins
+    got = oracle.wta(paths, min_d, roi_x, 0.95)
+    np.testing.assert_array_equal(got.view(np.uint32), wta_numpy(paths, min_d, roi_x, 0.95).view(np.uint32))
+
+
+def test_wta_closed_forms():
+    H, W, D = 12, 12, 64
+    paths = [np.full((H, W, D), 50, np.uint8) for _ in range(4)]
+    s = paths[0][6, 6]
+    s[20] = 0                            # symmetric minimum -> d* + 0.5
+    s[21] = 0                            # tie at 20/21 -> d* = 20, c3 = c2 -> uniqueness fails
+    out = oracle.wta(paths, 0, 0, 0.95)
+    assert out[6, 6] == INV
+    paths[0][6, 6, 21] = 50
+    out = oracle.wta(paths, 0, 0, 0.95)
+    assert out[6, 6] == np.float32(20.5)
+    paths[0][6, 6, 19] = 5               # S(19) = 155, S(20) = 150, S(21) = 200
+    out = oracle.wta(paths, 0, 0, 0.0)   # uniqueness 0: 5 <= 150 * 1.0 -> invalid
+    assert out[6, 6] == INV
+    out = oracle.wta(paths, 0, 0, 0.99)  # 5 > 150 * 0.01 -> valid, bias below 0.5
+    c1, c2, c3 = 155.0, 150.0, 200.0
+    den = c1 + c3 - 2 * c2
+    assert out[6, 6] == np.float32(20 + np.float32(c1 - c3 + den) / np.float32(2 * den))
+    assert (out[:5] == INV).all() and (out[:, :5] == INV).all() and (out[H - 4:] == INV).all()
+
+
+def test_lrcheck_cases():
+    H, W = 3, 20
+    dl = np.full((H, W), INV, np.float32)
+    dr = np.full((H, W), INV, np.float32)
+    dl[0, 10] = 4.4; dr[0, 6] = 4.9           # xr = round(5.6) = 6, |diff| 0.5 -> average
+    dl[1, 10] = 4.5; dr[1, 6] = 6.0           # xr = round(5.5) = 6 (half away), diff 1.5 -> both invalid
+    dl[2, 2] = 5.0                            # xr = -3 -> invalid
+    l, r = oracle.lrcheck(dl, dr, 1.0)
+    assert l[0, 10] == np.float32((np.float32(4.4) + np.float32(4.9)) / 2)
+    assert l[1, 10] == INV and r[1, 6] == INV
+    assert r[0, 6] == np.float32(4.9)
+    assert l[2, 2] == INV
+
+
+def test_lrcheck_numpy():
+    rng = np.random.default_rng(3)
+    H, W = 9, 50
+    dl = rng.uniform(0, 20, size=(H, W)).astype(np.float32)
+    dr = rng.uniform(0, 20, size=(H, W)).astype(np.float32)
+    dl[rng.random((H, W)) < 0.2] = INV
+    dr[rng.random((H, W)) < 0.2] = INV
+    l, r = oracle.lrcheck(dl, dr, 1.0)
+    el, er = dl.copy(), dr.copy()
+    for y in range(H):
+        for x in range(W):
+            d = dl[y, x]
+            if d == INV:
+                continue
+            xrf = np.float32(x) - d
+            xr = int(float(xrf) + 0.5) if xrf > 0 else int(float(xrf) - 0.5)
+            if xr < 0 or xr >= W:
+                el[y, x] = INV
+                continue
+            if abs(d - dr[y, xr]) > np.float32(1.0):
+                el[y, x] = INV
+                er[y, xr] = INV
+            else:
+                el[y, x] = (d + dr[y, xr]) / np.float32(2.0)
+    np.testing.assert_array_equal(l.view(np.uint32), el.view(np.uint32))
+    np.testing.assert_array_equal(r.view(np.uint32), er.view(np.uint32))
+
+
+def test_median_matches_scipy():
+    rng = np.random.default_rng(8)
+    d = rng.uniform(0, 64, size=(23, 31)).astype(np.float32)
+    d[rng.random(d.shape) < 0.3] = INV
+    d[4, 4] = d[4, 5]                       # duplicates
+    got = oracle.median3x3(d)
+    padded = np.pad(d, ((0, 1), (0, 1)), constant_values=INV)   # outside right/bottom = INVALID
+    exp = ndimage.median_filter(padded, size=3, mode="constant", cval=INV)[:-1, :-1]
+    exp[0, :] = d[0, :]
+    exp[:, 0] = d[:, 0]
+    np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("gen,kw,max_bad", [
+    (scenes.cone_like, {}, 0.02),
+    (scenes.driving, {}, 0.02),
+])
+def test_pipeline_recovers_ground_truth(gen, kw, max_bad):
+    p = gen(0, **kw)
+    dl, dr = oracle.match(p.left, p.right, 0, p.d_range)
+    valid = dl != INV
+    assert valid.mean() > 0.8
+    # the reference's parabola carries a +0.5 bias; compare after removing it
+    err = np.abs(dl[valid] - 0.5 - p.disparity[valid])
+    assert (err > 1.0).mean() < max_bad
+    assert (dr != INV).mean() > 0.7
+
+
+def test_pipeline_shift_invariance():
+    """A pure horizontal shift by delta gives disparity delta + 0.5 almost
+    everywhere (symmetric cost minimum -> the +0.5 bias)."""
+    rng = np.random.default_rng(0)
+    tex = scenes.value_noise(rng, 80, 200)
+    base = np.clip(np.rint(tex), 0, 255).astype(np.uint8)
+    delta = 13
+    left = base[:, :-delta].copy()          # left[x] = base[x]
+    right = base[:, delta:].copy()          # right[x - delta] = base[x] = left[x]
+    dl, _ = oracle.match(left, right, 0, 64, do_lr_check=False, do_median=False)
+    inner = dl[8:-8, 40:-8]
+    ok = inner[inner != INV]
+    assert ok.size > 0.9 * inner.size
+    assert (np.abs(ok - (delta + 0.5)) < 0.51).mean() > 0.97
+
+
+def test_pipeline_is_deterministic():
+    p = scenes.cone_like(1, H=60, W=90)
+    a = oracle.match(p.left, p.right, 0, 64)
+    b = oracle.match(p.left, p.right, 0, 64)
+    np.testing.assert_array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+    np.testing.assert_array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
