@@ -49,6 +49,13 @@ class SgmRoi(ctypes.Structure):
                 ("w", ctypes.c_int32), ("h", ctypes.c_int32)]
 
 
+class SgmPairDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("min_disparity", ctypes.c_int32), ("disp_range", ctypes.c_int32),
+                ("d_left", ctypes.c_void_p), ("d_right", ctypes.c_void_p),
+                ("d_disp_left", ctypes.c_void_p), ("d_disp_right", ctypes.c_void_p)]
+
+
 class SgmError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"SGM error {code}: {message}")
@@ -79,6 +86,13 @@ SIGNATURES = {
     "sgm_invalid_float": (F32, []),
     "sgm_invalid_short": (ctypes.c_int16, []),
     "sgm_level_range": (ctypes.c_int16, []),
+    "sgm_batch_create": (P, [ctypes.POINTER(SgmPairDesc), I32, ctypes.POINTER(SgmOption)]),
+    "sgm_batch_run": (ctypes.c_int, [P, P]),
+    "sgm_batch_set_timing": (ctypes.c_int, [P, I32]),
+    "sgm_batch_stage_times": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float),
+                                             ctypes.POINTER(I32)]),
+    "sgm_batch_device_bytes": (ctypes.c_size_t, [P]),
+    "sgm_batch_destroy": (None, [P]),
     "sgm_stage_census": (ctypes.c_int, [P, I32, I32, I32, I32, P, P]),
     "sgm_stage_cost": (ctypes.c_int, [P, P, I32, I32, I32, I32, I32, I32, P, P]),
     "sgm_stage_aggregate": (ctypes.c_int, [P, I32, I32, I32, I32, I32, I32, PP4, P]),

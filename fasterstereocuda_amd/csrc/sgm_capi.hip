@@ -35,6 +35,14 @@ int fail(std::string* dst, int code, const std::string& msg) {
             return fail((dst), SGM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+}  // namespace
+
+namespace fsgm {
+int set_global_error(int code, const std::string& msg) { return fail(nullptr, code, msg); }
+}  // namespace fsgm
+
+namespace {
+
 inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 bool valid_range(int D) { return D == 64 || D == 128 || D == 256 || D == 512; }
@@ -228,16 +236,7 @@ int run_pipeline(sgm_handle* h, bool want_right, hipStream_t s) {
     HIP_TRY(&h->err, launch_wta(ctw, s));
     mark("WTA/SubPixel:");
     HIP_TRY(&h->err, launch_post(full, s));
-    if (want_right) {
-        const size_t mapbytes = (size_t)h->W * h->H * sizeof(float);
-        if (h->opt.do_lr_check) {
-            HIP_TRY(&h->err, hipMemcpyAsync(h->view[1].out, h->view[1].raw, mapbytes,
-                                            hipMemcpyDeviceToDevice, s));
-            HIP_TRY(&h->err, launch_right_invalidate(full, s));
-        } else {
-            HIP_TRY(&h->err, launch_fill(h->view[1].out, (size_t)h->W * h->H, kInvalidDisp, s));
-        }
-    }
+    if (want_right && h->opt.do_lr_check) HIP_TRY(&h->err, launch_right_invalidate(full, s));
     mark("LRCheck/Filter:");
     return SGM_OK;
 }
@@ -404,6 +403,207 @@ float sgm_invalid_float(void) { return SGM_INVALID_FLOAT; }
 int16_t sgm_invalid_short(void) { return SGM_INVALID_SHORT; }
 int16_t sgm_level_range(void) { return SGM_LEVEL_RANGE; }
 
+// --------------------------------------------------------------- batches
+struct sgm_batch {
+    int device = -1;
+    sgm_option opt{};
+    void* arena = nullptr;
+    size_t arena_size = 0;
+    void* plan_block = nullptr;
+    StagePlan plan;
+    bool want_right = false;
+    bool timing = false;
+    std::vector<hipEvent_t> pool;     // free events
+    std::vector<hipEvent_t> marks;    // 6 per recorded run
+    std::string err;
+};
+
+namespace {
+
+hipError_t take_event(sgm_batch* b, hipEvent_t* e) {
+    if (!b->pool.empty()) {
+        *e = b->pool.back();
+        b->pool.pop_back();
+        return hipSuccess;
+    }
+    return hipEventCreate(e);
+}
+
+void free_batch(sgm_batch* b) {
+    if (b->device >= 0) (void)hipSetDevice(b->device);
+    for (hipEvent_t e : b->marks) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->pool) (void)hipEventDestroy(e);
+    if (b->arena) (void)hipFree(b->arena);
+    if (b->plan_block) (void)hipFree(b->plan_block);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+sgm_batch* sgm_batch_create(const sgm_pair_desc* pairs, int32_t n, const sgm_option* opt) {
+    std::string err;
+    if (!pairs || n <= 0) { fail(nullptr, SGM_E_INVALID_ARG, "empty batch"); return nullptr; }
+    if (check_option(opt, &err)) return nullptr;
+    for (int i = 0; i < n; ++i) {
+        const sgm_pair_desc& p = pairs[i];
+        if (p.width <= 0 || p.height <= 0 || !valid_range(p.disp_range) ||
+            std::max(0, p.min_disparity) >= p.width) {
+            fail(nullptr, SGM_E_INVALID_ARG, "pair " + std::to_string(i) + ": bad size or range");
+            return nullptr;
+        }
+        if (!p.d_left || !p.d_right || !p.d_disp_left) {
+            fail(nullptr, SGM_E_INVALID_ARG, "pair " + std::to_string(i) + ": NULL pointer");
+            return nullptr;
+        }
+    }
+    sgm_batch* b = new (std::nothrow) sgm_batch();
+    if (!b) { fail(nullptr, SGM_E_INVALID_ARG, "out of host memory"); return nullptr; }
+    b->opt = *opt;
+    if (hipGetDevice(&b->device) != hipSuccess) {
+        fail(nullptr, SGM_E_NO_DEVICE, "no HIP device");
+        delete b;
+        return nullptr;
+    }
+    const bool lr = opt->do_lr_check != 0;
+    // arena: per pair two census maps; cost + 4 paths + raw map per matched view
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const size_t npix = (size_t)pairs[i].width * pairs[i].height;
+        const size_t ncell = npix * pairs[i].disp_range;
+        total += 2 * align256(npix * 8) + (lr ? 2 : 1) * (5 * align256(ncell) + align256(npix * 4));
+    }
+    if (hipMalloc(&b->arena, total) != hipSuccess) {
+        fail(nullptr, SGM_E_HIP, "hipMalloc of " + std::to_string(total) + " bytes failed");
+        free_batch(b);
+        return nullptr;
+    }
+    b->arena_size = total;
+    uint8_t* cur = static_cast<uint8_t*>(b->arena);
+    auto take = [&](size_t bytes) { uint8_t* r = cur; cur += align256(bytes); return r; };
+    std::vector<SgmJob> left_jobs, right_jobs;
+    std::vector<SgmPost> posts;
+    for (int i = 0; i < n; ++i) {
+        const sgm_pair_desc& p = pairs[i];
+        const size_t npix = (size_t)p.width * p.height, ncell = npix * p.disp_range;
+        uint64_t* cs[2] = {reinterpret_cast<uint64_t*>(take(npix * 8)),
+                           reinterpret_cast<uint64_t*>(take(npix * 8))};
+        SgmPost post{};
+        for (int v = 0; v < (lr ? 2 : 1); ++v) {
+            SgmJob j{};
+            j.img = v ? p.d_right : p.d_left;
+            j.census = cs[v];
+            j.census_other = cs[1 - v];
+            j.cost = take(ncell);
+            for (int r = 0; r < 4; ++r) j.path[r] = take(ncell);
+            j.disp = reinterpret_cast<float*>(take(npix * 4));
+            j.W = p.width; j.H = p.height; j.D = p.disp_range; j.min_d = p.min_disparity;
+            j.roi_x = std::max(0, p.min_disparity);
+            j.is_right = v;
+            j.cs_type = opt->cs_type;
+            j.p1 = opt->p1; j.p2 = opt->p2_init;
+            j.uniqueness = opt->unique_threshold;
+            (v ? right_jobs : left_jobs).push_back(j);
+            (v ? post.raw_r : post.raw_l) = j.disp;
+        }
+        if (!lr) {
+            // the right census is still needed by the left cost: a census-only job
+            SgmJob j = left_jobs.back();
+            j.img = p.d_right; j.census = cs[1]; j.census_other = cs[0]; j.is_right = 1;
+            right_jobs.push_back(j);
+        }
+        post.out_l = p.d_disp_left;
+        post.out_r = p.d_disp_right;
+        post.W = p.width; post.H = p.height;
+        post.do_lr = lr ? 1 : 0;
+        post.do_median = opt->do_median_filter ? 1 : 0;
+        post.lr_thr = opt->lr_threshold;
+        posts.push_back(post);
+        b->want_right = b->want_right || p.d_disp_right != nullptr;
+    }
+    std::vector<SgmJob> jobs = left_jobs;
+    jobs.insert(jobs.end(), right_jobs.begin(), right_jobs.end());
+    if (upload_plan(jobs, posts, &b->plan, &b->plan_block, &b->err)) {
+        g_last_error = b->err;
+        free_batch(b);
+        return nullptr;
+    }
+    if (!lr) {
+        // only the left jobs (the first n) go through cost .. wta
+        StagePlan& P = b->plan;
+        int nc = 0, na = 0, nw = 0;
+        for (int i = 0; i < n; ++i) {
+            nc += cost_items(jobs[i]); na += agg_items(jobs[i]); nw += wta_items(jobs[i]);
+        }
+        P.n_cost = nc; P.n_agg = na; P.n_wta = nw;
+        // find_job over the first n jobs still works: their offsets come first
+        b->plan.njobs_ctw = n;
+    }
+    return b;
+}
+
+int sgm_batch_run(sgm_batch* b, void* stream) {
+    if (!b) return fail(nullptr, SGM_E_INVALID_ARG, "batch is NULL");
+    HIP_TRY(&b->err, hipSetDevice(b->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t ev[6] = {};
+    if (b->timing)
+        for (int k = 0; k < 6; ++k) HIP_TRY(&b->err, take_event(b, &ev[k]));
+    auto mark = [&](int k) -> hipError_t {
+        if (!b->timing) return hipSuccess;
+        b->marks.push_back(ev[k]);
+        return hipEventRecord(ev[k], s);
+    };
+    StagePlan ctw = b->plan;
+    if (b->plan.njobs_ctw > 0) ctw.njobs = b->plan.njobs_ctw;
+    HIP_TRY(&b->err, mark(0));
+    HIP_TRY(&b->err, launch_census(b->plan, s));
+    HIP_TRY(&b->err, mark(1));
+    HIP_TRY(&b->err, launch_cost(ctw, s));
+    HIP_TRY(&b->err, mark(2));
+    HIP_TRY(&b->err, launch_aggregate(ctw, s));
+    HIP_TRY(&b->err, mark(3));
+    HIP_TRY(&b->err, launch_wta(ctw, s));
+    HIP_TRY(&b->err, mark(4));
+    HIP_TRY(&b->err, launch_post(b->plan, s));
+    if (b->want_right && b->opt.do_lr_check) HIP_TRY(&b->err, launch_right_invalidate(b->plan, s));
+    HIP_TRY(&b->err, mark(5));
+    return SGM_OK;
+}
+
+int sgm_batch_set_timing(sgm_batch* b, int32_t on) {
+    if (!b) return fail(nullptr, SGM_E_INVALID_ARG, "batch is NULL");
+    b->timing = on != 0;
+    return SGM_OK;
+}
+
+int sgm_batch_stage_times(sgm_batch* b, float* ms, int32_t* runs) {
+    if (!b || !ms) return fail(nullptr, SGM_E_INVALID_ARG, "NULL argument");
+    HIP_TRY(&b->err, hipSetDevice(b->device));
+    for (int k = 0; k < 5; ++k) ms[k] = 0.f;
+    const int nrun = (int)(b->marks.size() / 6);
+    if (nrun > 0) HIP_TRY(&b->err, hipEventSynchronize(b->marks.back()));
+    for (int r = 0; r < nrun; ++r)
+        for (int k = 0; k < 5; ++k) {
+            float t = 0.f;
+            HIP_TRY(&b->err, hipEventElapsedTime(&t, b->marks[6 * r + k], b->marks[6 * r + k + 1]));
+            ms[k] += t;
+        }
+    b->pool.insert(b->pool.end(), b->marks.begin(), b->marks.end());
+    b->marks.clear();
+    if (runs) *runs = nrun;
+    return SGM_OK;
+}
+
+size_t sgm_batch_device_bytes(const sgm_batch* b) { return b ? b->arena_size : 0; }
+
+void sgm_batch_destroy(sgm_batch* b) {
+    if (b) free_batch(b);
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------ stage entries
 namespace {
 
@@ -515,13 +715,7 @@ int sgm_stage_post(const float* d_raw_left, const float* d_raw_right, int32_t wi
     if (int rc = upload_plan(jobs, posts, &t.plan, &t.block, nullptr)) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIP_TRY(nullptr, launch_post(t.plan, s));
-    if (d_out_right) {
-        if (do_lr_check) {
-            HIP_TRY(nullptr, hipMemcpyAsync(d_out_right, d_raw_right, (size_t)width * height * 4,
-                                            hipMemcpyDeviceToDevice, s));
-            HIP_TRY(nullptr, launch_right_invalidate(t.plan, s));
-        }
-    }
+    if (d_out_right && do_lr_check) HIP_TRY(nullptr, launch_right_invalidate(t.plan, s));
     HIP_TRY(nullptr, hipStreamSynchronize(s));
     return SGM_OK;
 }

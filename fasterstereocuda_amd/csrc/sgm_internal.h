@@ -61,6 +61,7 @@ struct StagePlan {
     SgmPost* posts = nullptr;     // device
     int* offs_post = nullptr;
     int njobs = 0, nposts = 0;
+    int njobs_ctw = 0;            // if > 0: jobs [0, njobs_ctw) run cost..wta
     int n_census = 0, n_cost = 0, n_agg = 0, n_wta = 0, n_post = 0;
     int dmax = 64;
 };

@@ -482,6 +482,10 @@ __global__ __launch_bounds__(256) void post_kernel(const SgmPost* __restrict__ p
     const int tx = tid & 15, ty = tid >> 4;
     const int x = x0 + tx, y = y0 + ty;
     if (x >= P.W || y >= P.H) return;
+    // the right output starts as the right map (LR invalidations follow in a
+    // second launch) or, without the LR check, as all INVALID
+    if (P.out_r)
+        P.out_r[(int64_t)y * P.W + x] = P.do_lr ? P.raw_r[(int64_t)y * P.W + x] : kInvalidDisp;
     float v = t[ty + 1][tx + 1];
     if (P.do_median && x > 0 && y > 0)
         v = median9(t[ty][tx], t[ty][tx + 1], t[ty][tx + 2], t[ty + 1][tx], t[ty + 1][tx + 1],

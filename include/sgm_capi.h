@@ -109,6 +109,55 @@ float sgm_invalid_float(void);
 int16_t sgm_invalid_short(void);
 int16_t sgm_level_range(void);
 
+/* ---- batched matching (no reference counterpart) ---------------------------
+ * Many pairs, each with its own size and disparity range, matched by one
+ * launch per stage.  Pointers in the descriptors are device pointers and are
+ * bound at creation; the batch owns every intermediate buffer.  Per pair the
+ * output is byte-identical to sgm_match on a handle initialised with the
+ * same size, range and options. */
+typedef struct sgm_pair_desc {
+    int32_t width, height, min_disparity, disp_range;
+    const uint8_t* d_left;
+    const uint8_t* d_right;
+    float* d_disp_left;
+    float* d_disp_right;            /* may be NULL */
+} sgm_pair_desc;
+
+typedef struct sgm_batch sgm_batch;
+
+/* Returns NULL on failure (message in sgm_last_error(NULL)). */
+sgm_batch* sgm_batch_create(const sgm_pair_desc* pairs, int32_t n, const sgm_option* opt);
+/* Enqueues the whole pipeline for every pair on `stream`. */
+int sgm_batch_run(sgm_batch* b, void* stream);
+/* With timing on, every run records HIP events between its stages. */
+int sgm_batch_set_timing(sgm_batch* b, int32_t on);
+/* Sums of the per-stage times (ms) of the runs recorded since the last call:
+ * ms[0..4] = census, cost, aggregate, wta, post.  Waits for those runs. */
+int sgm_batch_stage_times(sgm_batch* b, float* ms, int32_t* runs);
+size_t sgm_batch_device_bytes(const sgm_batch* b);
+void sgm_batch_destroy(sgm_batch* b);
+
+/* ---- differentiable FP32 SGM (no reference counterpart) ------------------
+ * Float cost volume C[B][H][W][D], per-pixel penalties P1, P2 [B][H][W],
+ * soft-argmin disparity with temperature T.  Four directions over the whole
+ * image with the textbook boundary (no d-1 term at d = 0, no d+1 term at
+ * d = D-1).  D must be 64, 128 or 256.  All buffers are device memory owned
+ * by the caller; sizes in floats come from the two query functions.  The
+ * forward stores S (needed by the backward) and per-direction checkpoints
+ * every 32 scanline steps instead of the full per-direction volumes. */
+size_t sgm_dsgm_checkpoint_floats(int32_t B, int32_t H, int32_t W, int32_t D);
+size_t sgm_dsgm_workspace_floats(int32_t B, int32_t H, int32_t W, int32_t D);
+int sgm_dsgm_forward(const float* d_cost, const float* d_p1, const float* d_p2, int32_t B,
+                     int32_t H, int32_t W, int32_t D, float temperature, float* d_S,
+                     float* d_disp, float* d_ckpt, float* d_work, void* stream);
+/* Gradients of a scalar loss given dLoss/d disp.  d_grad_cost doubles as
+ * scratch for dLoss/dS until the last kernel writes the result. */
+int sgm_dsgm_backward(const float* d_cost, const float* d_p1, const float* d_p2,
+                      const float* d_S, const float* d_disp, const float* d_ckpt,
+                      const float* d_grad_disp, int32_t B, int32_t H, int32_t W, int32_t D,
+                      float temperature, float* d_grad_cost, float* d_grad_p1,
+                      float* d_grad_p2, float* d_work, void* stream);
+
 /* ---- stage entry points (device pointers; used by the parity tests) ----
  * Each runs one stage of the plain pipeline over one view.  Layouts:
  * census u64 [H][W]; cost and path volumes u8 [H][W][D] (d fastest). */

@@ -158,3 +158,144 @@ def stages(left: np.ndarray, right: np.ndarray, min_d: int, D: int, cs_type: int
     out["lr_l"], out["lr_r"] = lrcheck(wl, wr, lr_threshold)
     out["median"] = median3x3(out["lr_l"])
     return out
+
+
+# ---------------------------------------------------------------- diff-SGM
+_dlib = None
+
+
+def dlib():
+    """FP64 reference of the differentiable SGM (oracle/dsgm_oracle.cpp)."""
+    global _dlib
+    if _dlib is None:
+        path = os.path.join(_HERE, "libdsgm_oracle.so")
+        if not os.path.exists(path):
+            build(force=True)
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        Dd = ctypes.c_double
+        L.dsgm_forward.argtypes = [P, P, P, P, I, I, I, Dd, P, P]
+        L.dsgm_forward.restype = Dd
+        L.dsgm_backward.argtypes = [P, P, P, P, I, I, I, Dd, P, P, P]
+        L.dsgm_backward.restype = Dd
+        _dlib = L
+    return _dlib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def dsgm_forward(cost, p1, p2, gt=None, temperature=0.1):
+    """Returns (S [H,W,D], disp [H,W], loss) in float64."""
+    cost, p1, p2 = _f64(cost), _f64(p1), _f64(p2)
+    H, W, D = cost.shape
+    S = np.empty((H, W, D), np.float64)
+    disp = np.empty((H, W), np.float64)
+    g = None if gt is None else _f64(gt)
+    loss = dlib().dsgm_forward(_p(cost), _p(p1), _p(p2), _p(g), H, W, D, temperature,
+                               _p(S), _p(disp))
+    return S, disp, loss
+
+
+def dsgm_backward(cost, p1, p2, gt, temperature=0.1):
+    """Gradient of the mean L1 loss.  Returns (gC, gP1, gP2, loss) in float64."""
+    cost, p1, p2, gt = _f64(cost), _f64(p1), _f64(p2), _f64(gt)
+    H, W, D = cost.shape
+    gC = np.empty((H, W, D), np.float64)
+    g1 = np.empty((H, W), np.float64)
+    g2 = np.empty((H, W), np.float64)
+    loss = dlib().dsgm_backward(_p(cost), _p(p1), _p(p2), _p(gt), H, W, D, temperature,
+                                _p(gC), _p(g1), _p(g2))
+    return gC, g1, g2, loss
+
+
+# ---------------------------------------------------------------- pyramid
+INVALID_SHORT = -32768
+
+
+def _plib():
+    L = lib()
+    if not getattr(L, "_pyr_ready", False):
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        F = ctypes.c_float
+        L.orc_zoom_half.argtypes = [P, I, I, P]
+        L.orc_init_next.argtypes = [P, I, I, I, P]
+        L.orc_cost_init.argtypes = [P, P, P, I, I, I, I, P]
+        L.orc_aggregate_init.argtypes = [P, P, I, I, I, I, I, I, P]
+        L.orc_wta_init.argtypes = [P, P, P, P, P, I, I, I, F, P]
+        L.orc_match_init.argtypes = [P, P, P, I, I, I, ctypes.POINTER(Options), P]
+        L.orc_match_init.restype = I
+        L.orc_pyramid.argtypes = [P, P, I, I, I, I, I, I, ctypes.POINTER(Options), P]
+        L.orc_pyramid.restype = I
+        L._pyr_ready = True
+    return L
+
+
+def zoom_half(img):
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape
+    out = np.empty((h // 2, w // 2), np.uint8)
+    _plib().orc_zoom_half(_p(img), w, h, _p(out))
+    return out
+
+
+def init_next(coarse, w, h, level_range):
+    coarse = np.ascontiguousarray(coarse, np.float32)
+    out = np.empty((h, w), np.int16)
+    _plib().orc_init_next(_p(coarse), w, h, level_range, _p(out))
+    return out
+
+
+def cost_init(cl, cr, init, level_range, cs_type=1):
+    H, W = cl.shape
+    out = np.empty((H, W, level_range), np.uint8)
+    _plib().orc_cost_init(_p(cl), _p(cr), _p(np.ascontiguousarray(init, np.int16)), W, H,
+                          level_range, cs_type, _p(out))
+    return out
+
+
+def aggregate_init(cost_vol, init, direction, p1, p2, fill=0):
+    H, W, D = cost_vol.shape
+    out = np.full((H, W, D), fill, np.uint8)
+    _plib().orc_aggregate_init(_p(cost_vol), _p(np.ascontiguousarray(init, np.int16)), W, H, D,
+                               direction, p1, p2, _p(out))
+    return out
+
+
+def wta_init(paths, init, uniqueness):
+    H, W, D = paths[0].shape
+    out = np.empty((H, W), np.float32)
+    _plib().orc_wta_init(*[_p(p) for p in paths], _p(np.ascontiguousarray(init, np.int16)), W, H,
+                         D, uniqueness, _p(out))
+    return out
+
+
+def match_init(left, right, init, level_range, cs_type=1, p1=10, p2=150, uniqueness=0.95,
+               do_median=True):
+    left = np.ascontiguousarray(left, np.uint8)
+    right = np.ascontiguousarray(right, np.uint8)
+    H, W = left.shape
+    opt = Options(cs_type, p1, p2, uniqueness, 0, 1.0, int(do_median))
+    out = np.empty((H, W), np.float32)
+    rc = _plib().orc_match_init(_p(left), _p(right), _p(np.ascontiguousarray(init, np.int16)), W,
+                                H, level_range, ctypes.byref(opt), _p(out))
+    if rc:
+        raise RuntimeError(f"orc_match_init failed with {rc}")
+    return out
+
+
+def pyramid(left, right, min_disp, max_disp, num_layers, level_range, cs_type=1, p1=10, p2=150,
+            uniqueness=0.95, do_lr_check=True, lr_threshold=1.0, do_median=True):
+    left = np.ascontiguousarray(left, np.uint8)
+    right = np.ascontiguousarray(right, np.uint8)
+    H, W = left.shape
+    opt = Options(cs_type, p1, p2, uniqueness, int(do_lr_check), lr_threshold, int(do_median))
+    out = np.empty((H, W), np.float32)
+    rc = _plib().orc_pyramid(_p(left), _p(right), W, H, min_disp, max_disp, num_layers,
+                             level_range, ctypes.byref(opt), _p(out))
+    if rc:
+        raise RuntimeError(f"orc_pyramid failed with {rc}")
+    return out

@@ -25,6 +25,12 @@
  *       (cusgm_cc.cu:288); that pixel keeps the left-to-right cost the same
  *       buffer held: kept
  *   A17 with the LR check off the right map is INVALID everywhere
+ *   A6  pyramid P2 divisor abs(b' - b + 1) is clamped to >= 1
+ *   A11 vertical pyramid paths seed the previous base disparity from row 0
+ *       for both directions (cusgm_ca.cu:1291): kept
+ *   A15 the pyramid window (LEVEL_RANGE) is a parameter; CostAdjust uses it
+ *   A18 init propagation with an odd level width clamps the coarse column
+ *       (the reference reads one element past the coarse row)
  */
 #include "sgm_oracle.h"
 
@@ -301,6 +307,233 @@ void orc_median3x3(const float* in, float* out, int W, int H) {
             out[idx2(x, y, W)] = v[4];
         }
     }
+}
+
+/* ------------------------------------------------------------ pyramid --- */
+
+/* FasterStereoCuda.cpp:114-136 (Zoom by 2): dst[i][j] = src[2i][2j]. */
+void orc_zoom_half(const uint8_t* src, int w, int h, uint8_t* dst) {
+    const int wd = w / 2, hd = h / 2;
+    for (int i = 0; i < hd; ++i)
+        for (int j = 0; j < wd; ++j) dst[(size_t)i * wd + j] = src[(size_t)(2 * i) * w + 2 * j];
+}
+
+/* FasterStereoCuda.cpp:138-166.  next is w x h; coarse is (w/2) x (h/2).
+ * Even rows (but the last): b = (short)(2 d - LR/2) from the coarse pixel
+ * (i/2, j/2), INVALID -> -32768; odd rows and the last row copy the row
+ * above.  A18: the coarse column is clamped to the coarse width. */
+void orc_init_next(const float* coarse, int w, int h, int level_range, int16_t* next) {
+    const int half = level_range >> 1;
+    const int wc = w >> 1;
+    for (int i = 0; i < h; ++i) {
+        int16_t* row = next + (size_t)i * w;
+        for (int j = 0; j < w; ++j) {
+            if (i % 2 == 1 || i == h - 1) {
+                row[j] = next[(size_t)(i - 1) * w + j];
+            } else {
+                const float d = coarse[(size_t)(i >> 1) * wc + std::min(j >> 1, wc - 1)];
+                row[j] = (d != ORC_INVALID_DISP) ? (int16_t)(2 * d - half) : (int16_t)ORC_INVALID_SHORT;
+            }
+        }
+    }
+}
+
+/* cusgm_cc.cu:14-21: (uint8)expf(4*|t - LR/2| / (LR/2)) with integer
+ * division, so the exponent is an integer 0..4: e^k truncated. */
+static uint8_t cost_adjust(int t, int level_range) {
+    static const uint8_t table[5] = {1, 2, 7, 20, 54};   // floor(e^k), k = 0..4
+    const int half = level_range / 2;
+    const int k = 4 * std::abs(t - half) / half;
+    return table[std::min(k, 4)];
+}
+
+/* Cost with initial disparity, left view (cusgm_cc.cu:173-237 for 9x7,
+ * :294-360 for 5x5): for t in [0, LR): xr = x - t - b(x);
+ * 0 < xr < W -> popc(cl[x] ^ cr[xr]) + CostAdjust(t); otherwise 0xFF (9x7)
+ * or 127 (5x5). */
+void orc_cost_init(const uint64_t* cl, const uint64_t* cr, const int16_t* init, int W, int H,
+                   int level_range, int cs_type, uint8_t* cost) {
+    const int D = level_range;
+    const uint8_t invalid = cs_type ? 0xFF : (uint8_t)ORC_INVALID_COST;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const int b = init[(size_t)y * W + x];
+            for (int t = 0; t < D; ++t) {
+                const int xr = x - t - b;
+                uint8_t c = invalid;
+                if (xr > 0 && xr < W)
+                    c = (uint8_t)(popcount64(cl[(size_t)y * W + x] ^ cr[(size_t)y * W + xr]) +
+                                  cost_adjust(t, level_range));
+                cost[idx3(x, y, t, W, D)] = c;
+            }
+        }
+}
+
+/* Aggregation over a per-pixel window (cusgm_ca.cu:1181-1339), uint32:
+ *   first pixel: L = C (A8: stored), m = min C, b' = b(p0) (A11: vertical
+ *     directions take b' from row 0 whichever end they start at)
+ *   b(p) INVALID: m = 127, nothing else changes, nothing is stored
+ *   else: i1 = t + b - 1 - b', i2 = t + b + 1 - b'
+ *     B- = i1 in [0, LR) ? L'[i1] + P1 : 127;  B+ = i2 in [0, LR) ? L'[i2] + P1 : 127
+ *     P2' = max(P1, P2 / max(1, |b' - b + 1|))          (A6)
+ *     L[t] = C + (min(L'[t], B-, B+, m + P2') - m)     (wraps mod 2^32)
+ *     store (uint8)L, m = min L, b' = b
+ * Pixels with an INVALID b keep whatever the output held (the WTA never
+ * reads them). */
+void orc_aggregate_init(const uint8_t* cost, const int16_t* init, int W, int H,
+                        int level_range, int dir, int p1, int p2, uint8_t* out) {
+    const int D = level_range;
+    const bool horiz = (dir == ORC_DIR_LR || dir == ORC_DIR_RL);
+    const bool fwd = (dir == ORC_DIR_UD || dir == ORC_DIR_LR);
+    const int nlines = horiz ? H : W, n = horiz ? W : H;
+#pragma omp parallel
+    {
+        std::vector<uint32_t> Lp(D), Ln(D);
+#pragma omp for schedule(static)
+        for (int line = 0; line < nlines; ++line) {
+            auto pix = [&](int s) -> size_t {
+                const int i = fwd ? s : n - 1 - s;
+                return horiz ? (size_t)line * W + i : (size_t)i * W + line;
+            };
+            size_t p = pix(0);
+            uint32_t m = 0xFFFFFFFFu;
+            for (int t = 0; t < D; ++t) {
+                Lp[t] = cost[p * D + t];
+                out[p * D + t] = (uint8_t)Lp[t];
+                m = std::min(m, Lp[t]);
+            }
+            int last_b = horiz ? init[p] : init[(size_t)line];
+            for (int s = 1; s < n; ++s) {
+                p = pix(s);
+                const int b = init[p];
+                if (b == ORC_INVALID_SHORT) { m = ORC_INVALID_COST; continue; }
+                const int div = std::max(1, std::abs(last_b - b + 1));
+                const uint32_t pen2 = (uint32_t)std::max(p1, p2 / div);
+                uint32_t mn = 0xFFFFFFFFu;
+                for (int t = 0; t < D; ++t) {
+                    const int i1 = t + b - 1 - last_b, i2 = t + b + 1 - last_b;
+                    const uint32_t bm = (i1 >= 0 && i1 < D) ? Lp[i1] + (uint32_t)p1 : ORC_INVALID_COST;
+                    const uint32_t bp = (i2 >= 0 && i2 < D) ? Lp[i2] + (uint32_t)p1 : ORC_INVALID_COST;
+                    const uint32_t v = std::min(std::min(Lp[t], bm), std::min(bp, m + pen2));
+                    const uint32_t l = (uint32_t)cost[p * D + t] + (v - m);
+                    Ln[t] = l;
+                    out[p * D + t] = (uint8_t)l;
+                    mn = std::min(mn, l);
+                }
+                m = mn;
+                last_b = b;
+                std::swap(Lp, Ln);
+            }
+        }
+    }
+}
+
+/* WTA with initial disparity (cusgm_ca.cu:3425-3512): border or INVALID b ->
+ * INVALID; d* = smallest argmin of S (A1); INVALID unless 2 <= d* <= LR-2;
+ * uniqueness; disp = (d* + b) + SubPixBias.  Rows [4*(H/4), H) are never
+ * written by the reference grid (A5: INVALID). */
+void orc_wta_init(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2, const uint8_t* p3,
+                  const int16_t* init, int W, int H, int level_range, float uniqueness,
+                  float* disp) {
+    const int D = level_range;
+    const int rows_covered = 4 * (H / 4);
+    const float unique_threshold = 1.0f - uniqueness;
+#pragma omp parallel
+    {
+        std::vector<uint32_t> S(D);
+#pragma omp for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                float& out = disp[(size_t)y * W + x];
+                out = ORC_INVALID_DISP;
+                if (y >= rows_covered) continue;
+                if (x > W - 5 || y > H - 5 || x < 5 || y < 5) continue;
+                const int b = init[(size_t)y * W + x];
+                if (b == ORC_INVALID_SHORT) continue;
+                const size_t base = ((size_t)y * W + x) * D;
+                uint32_t c2 = 0xFFFFFFFFu;
+                int ds = 0;
+                for (int d = 0; d < D; ++d) {
+                    S[d] = (uint32_t)p0[base + d] + p1[base + d] + p2[base + d] + p3[base + d];
+                    if (S[d] < c2) { c2 = S[d]; ds = d; }
+                }
+                if (ds <= 1 || ds >= D - 1) continue;
+                const float c1 = (float)S[ds - 1], c2f = (float)c2, c3 = (float)S[ds + 1];
+                const float a = c1 - c2f, bb = c3 - c2f;
+                const float uniq = (a < bb) ? a : bb;
+                if (uniq <= c2f * unique_threshold) continue;
+                const float t = c1 + c3 - 2.0f * c2f;
+                const float den = (1.0f < t) ? t : 1.0f;
+                out = (float)(ds + b) + (c1 - c3 + den) / (2.0f * den);
+            }
+    }
+}
+
+int orc_match_init(const uint8_t* left, const uint8_t* right, const int16_t* init, int W, int H,
+                   int level_range, const orc_options* opt, float* disp) {
+    if (!left || !right || !init || !disp || W <= 0 || H <= 0) return -1;
+    const size_t npix = (size_t)W * H, ncell = npix * level_range;
+    std::vector<uint64_t> cl(npix), cr(npix);
+    std::vector<uint8_t> cost(ncell);
+    std::vector<uint8_t> path[4];
+    for (auto& p : path) p.assign(ncell, 0);
+    orc_census(left, W, H, opt->cs_type, 0, cl.data());
+    orc_census(right, W, H, opt->cs_type, 1, cr.data());
+    orc_cost_init(cl.data(), cr.data(), init, W, H, level_range, opt->cs_type, cost.data());
+    for (int r = 0; r < 4; ++r)
+        orc_aggregate_init(cost.data(), init, W, H, level_range, r, opt->p1, opt->p2, path[r].data());
+    std::vector<float> raw(npix);
+    orc_wta_init(path[0].data(), path[1].data(), path[2].data(), path[3].data(), init, W, H,
+                 level_range, opt->uniqueness, raw.data());
+    if (opt->do_median) orc_median3x3(raw.data(), disp, W, H);
+    else std::memcpy(disp, raw.data(), npix * sizeof(float));
+    return 0;
+}
+
+int orc_pyramid(const uint8_t* left, const uint8_t* right, int W, int H, int min_disp,
+                int max_disp, int num_layers, int level_range, const orc_options* opt,
+                float* disp) {
+    if (num_layers < 1 || W <= 0 || H <= 0 || max_disp <= min_disp) return -1;
+    std::vector<std::vector<uint8_t>> L(num_layers), R(num_layers);
+    std::vector<int> lw(num_layers), lh(num_layers);
+    for (int i = 0; i < num_layers; ++i) { lw[i] = W >> i; lh[i] = H >> i; }
+    L[0].assign(left, left + (size_t)W * H);
+    R[0].assign(right, right + (size_t)W * H);
+    for (int i = 1; i < num_layers; ++i) {
+        L[i].resize((size_t)lw[i] * lh[i]);
+        R[i].resize((size_t)lw[i] * lh[i]);
+        orc_zoom_half(L[i - 1].data(), lw[i - 1], lh[i - 1], L[i].data());
+        orc_zoom_half(R[i - 1].data(), lw[i - 1], lh[i - 1], R[i].data());
+    }
+    // coarsest level: plain SGM over the scaled range, rounded up to 2^k >= 64
+    const int top = num_layers - 1, scale = 1 << top;
+    int dmin = min_disp / scale;
+    const int dmax = max_disp / scale;
+    // FasterStereoCuda.cpp:265-281 compares against the full-resolution width
+    int range = 64;
+    while (range < dmax - dmin) range <<= 1;
+    if (range > 2 * W) range >>= 1;
+    if (dmin >= W) dmin = std::max(W - range, -W);
+    orc_options o = *opt;
+    if (top > 0) o.uniqueness = 1.0f;
+    std::vector<float> cur((size_t)lw[top] * lh[top]);
+    if (top == 0) return orc_match(left, right, W, H, dmin, range, &o, disp, nullptr);
+    int rc = orc_match(L[top].data(), R[top].data(), lw[top], lh[top], dmin, range, &o, cur.data(), nullptr);
+    if (rc) return rc;
+    o.do_lr_check = 0;
+    for (int n = top - 1; n >= 0; --n) {
+        std::vector<int16_t> init((size_t)lw[n] * lh[n]);
+        orc_init_next(cur.data(), lw[n], lh[n], level_range, init.data());
+        o.uniqueness = (n == 0) ? opt->uniqueness : 1.0f;
+        std::vector<float> out((size_t)lw[n] * lh[n]);
+        rc = orc_match_init(L[n].data(), R[n].data(), init.data(), lw[n], lh[n], level_range, &o,
+                            out.data());
+        if (rc) return rc;
+        cur.swap(out);
+    }
+    std::memcpy(disp, cur.data(), (size_t)W * H * sizeof(float));
+    return 0;
 }
 
 /* Plain single-level pipeline in StereoCudaImpl::Match order

@@ -64,6 +64,28 @@ void orc_lrcheck(float* disp_l, const float* disp_r_in, float* disp_r_out,
 
 void orc_median3x3(const float* in, float* out, int W, int H);
 
+/* ---- coarse-to-fine pyramid (FasterStereoCuda) ---- */
+#define ORC_INVALID_SHORT (-32768)
+
+void orc_zoom_half(const uint8_t* src, int w, int h, uint8_t* dst);
+void orc_init_next(const float* coarse, int w, int h, int level_range, int16_t* next);
+void orc_cost_init(const uint64_t* cl, const uint64_t* cr, const int16_t* init, int W, int H,
+                   int level_range, int cs_type, uint8_t* cost);
+void orc_aggregate_init(const uint8_t* cost, const int16_t* init, int W, int H,
+                        int level_range, int dir, int p1, int p2, uint8_t* out);
+void orc_wta_init(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2, const uint8_t* p3,
+                  const int16_t* init, int W, int H, int level_range, float uniqueness,
+                  float* disp);
+/* One finer level: census, init cost, 4 paths, init WTA, [median]. */
+int orc_match_init(const uint8_t* left, const uint8_t* right, const int16_t* init, int W, int H,
+                   int level_range, const orc_options* opt, float* disp);
+/* Whole pyramid: num_layers levels, coarsest plain (with LR if opt->do_lr_check),
+ * finer levels searched in a window of level_range around the propagated
+ * disparity; uniqueness opt->uniqueness at level 0 and 1.0 above. */
+int orc_pyramid(const uint8_t* left, const uint8_t* right, int W, int H, int min_disp,
+                int max_disp, int num_layers, int level_range, const orc_options* opt,
+                float* disp);
+
 /* Whole plain pipeline (StereoCudaImpl::Match order).  disp_r may be NULL.
  * Returns 0 on success. */
 int orc_match(const uint8_t* left, const uint8_t* right, int W, int H,

@@ -160,7 +160,10 @@ def test_errors_are_reported(cuda):
     assert "remove_peaks" in st.last_error
     left = np.zeros((50, 100), np.uint8)
     out = np.empty((50, 100), np.float32)
-    assert not st.Match(left, left, out)          # not initialised
+    fresh = StereoCuda()
+    fresh.width, fresh.height = 100, 50
+    assert not fresh.Match(left, left, out)        # not initialised
+    assert "not initialised" in fresh.last_error
     assert st.Init(100, 50, 0, 64, CuSGMOption.benchmark())
     assert st.Match(left, left, out)
     assert st.GetDispartyRange() == 64
