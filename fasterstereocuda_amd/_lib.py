@@ -93,6 +93,22 @@ SIGNATURES = {
                                              ctypes.POINTER(I32)]),
     "sgm_batch_device_bytes": (ctypes.c_size_t, [P]),
     "sgm_batch_destroy": (None, [P]),
+    "sgm_dsgm_checkpoint_floats": (ctypes.c_size_t, [I32, I32, I32, I32]),
+    "sgm_dsgm_workspace_floats": (ctypes.c_size_t, [I32, I32, I32, I32]),
+    "sgm_dsgm_forward": (ctypes.c_int, [P, P, P, I32, I32, I32, I32, F32, P, P, P, P, P]),
+    "sgm_dsgm_backward": (ctypes.c_int, [P, P, P, P, P, P, P, I32, I32, I32, I32, F32, P, P, P,
+                                         P, P]),
+    "sgm_pyramid_create": (P, [I32, I32, I32, I32, I32, I32, ctypes.POINTER(SgmOption)]),
+    "sgm_pyramid_match": (ctypes.c_int, [P, P, P, P]),
+    "sgm_pyramid_match_device": (ctypes.c_int, [P, P, P, P, P]),
+    "sgm_pyramid_coarsest_range": (I32, [P, ctypes.POINTER(I32)]),
+    "sgm_pyramid_device_bytes": (ctypes.c_size_t, [P]),
+    "sgm_pyramid_destroy": (None, [P]),
+    "sgm_stage_zoom": (ctypes.c_int, [P, I32, I32, P, P]),
+    "sgm_stage_init_next": (ctypes.c_int, [P, I32, I32, I32, P, P]),
+    "sgm_stage_cost_init": (ctypes.c_int, [P, P, P, I32, I32, I32, I32, P, P]),
+    "sgm_stage_aggregate_init": (ctypes.c_int, [P, P, I32, I32, I32, I32, I32, PP4, P]),
+    "sgm_stage_wta_init": (ctypes.c_int, [PP4, P, I32, I32, I32, F32, P, P]),
     "sgm_stage_census": (ctypes.c_int, [P, I32, I32, I32, I32, P, P]),
     "sgm_stage_cost": (ctypes.c_int, [P, P, I32, I32, I32, I32, I32, I32, P, P]),
     "sgm_stage_aggregate": (ctypes.c_int, [P, I32, I32, I32, I32, I32, I32, PP4, P]),

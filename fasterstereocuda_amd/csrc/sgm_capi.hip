@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "sgm_capi.h"
+#include "pyr_internal.h"
 #include "sgm_internal.h"
 
 using namespace fsgm;
@@ -65,8 +66,10 @@ int check_option(const sgm_option* o, std::string* err) {
     return SGM_OK;
 }
 
+}  // namespace
+
 // Uploads jobs / posts and the per-stage item tables into one device block.
-int upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost>& posts,
+int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost>& posts,
                 StagePlan* plan, void** block, std::string* err) {
     const int nj = (int)jobs.size(), np = (int)posts.size();
     std::vector<int> oc(nj + 1, 0), ok(nj + 1, 0), oa(nj + 1, 0), ow(nj + 1, 0), op(np + 1, 0);
@@ -123,8 +126,6 @@ int upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost>& pos
     plan->dmax = dmax;
     return SGM_OK;
 }
-
-}  // namespace
 
 // Per-view device buffers of one pair.
 struct ViewBuffers {

@@ -81,9 +81,13 @@ def diff_sgm(cost: torch.Tensor, p1: torch.Tensor, p2: torch.Tensor,
 def census_cost_volume(left: torch.Tensor, right: torch.Tensor, D: int) -> torch.Tensor:
     """FP32 normalised Hamming cost (9x7 census, cost / 63) of uint8 pairs
     [B, H, W] -> [B, H, W, D], computed by the census / cost HIP kernels."""
+    # costs are integers 0..63: map them through a table of correctly rounded
+    # quotients so the volume is exactly cost / 63 in float32
+    import numpy as np
+    table = torch.from_numpy((np.arange(256, dtype=np.float32) / np.float32(63.0))).to(left.device)
     out = []
     for b in range(left.shape[0]):
         cl = stages.census(left[b].contiguous(), 1, False)
         cr = stages.census(right[b].contiguous(), 1, True)
-        out.append(stages.cost(cl, cr, 0, D, True, 1).to(torch.float32) / 63.0)
+        out.append(table[stages.cost(cl, cr, 0, D, True, 1).long()])
     return torch.stack(out)

@@ -1,0 +1,122 @@
+"""Coarse-to-fine pyramid matching (the FasterStereoCuda surface).
+
+``SgmPyramid`` exposes the engine's pyramid directly (any census / P1 / P2
+/ level range); ``FasterStereoCuda`` mirrors the reference class
+(FasterStereoCuda/FasterStereoCuda.h:24-127): Init with StereoOption1 and
+Match a pair into a float disparity map, with the reference's per-level
+option schedule (FasterStereoCuda.cpp:169-199: 5x5 census, P1 = 8, P2 = 32,
+uniqueness 0.95 at level 0 and 1.0 above, LR check at the coarsest level,
+3x3 median, dilation of the coarsest level, window of 32 disparities).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import last_error, lib
+from .stereo import CuSGMOption
+
+
+class SgmPyramid:
+    def __init__(self, width: int, height: int, min_disp: int, max_disp: int, num_layers: int,
+                 level_range: int, option: CuSGMOption):
+        opt = option.to_c()
+        self._p = lib().sgm_pyramid_create(width, height, min_disp, max_disp, num_layers,
+                                           level_range, ctypes.byref(opt))
+        if not self._p:
+            raise RuntimeError(f"sgm_pyramid_create failed: {last_error()}")
+        self.width, self.height = width, height
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p:
+            lib().sgm_pyramid_destroy(p)
+            self._p = None
+
+    @property
+    def coarsest(self):
+        m = ctypes.c_int32(0)
+        r = lib().sgm_pyramid_coarsest_range(self._p, ctypes.byref(m))
+        return m.value, r
+
+    @property
+    def device_bytes(self) -> int:
+        return lib().sgm_pyramid_device_bytes(self._p)
+
+    def match(self, left: np.ndarray, right: np.ndarray) -> np.ndarray:
+        left = np.ascontiguousarray(left, np.uint8)
+        right = np.ascontiguousarray(right, np.uint8)
+        out = np.empty((self.height, self.width), np.float32)
+        _lib.check(lib().sgm_pyramid_match(self._p, left.ctypes.data, right.ctypes.data,
+                                           out.ctypes.data))
+        return out
+
+    def match_device(self, d_left, d_right, d_disp, stream=None) -> None:
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(lib().sgm_pyramid_match_device(self._p, d_left.data_ptr(), d_right.data_ptr(),
+                                                  d_disp.data_ptr(), stream))
+
+
+@dataclass
+class StereoOption1:
+    """ste_opt1_t (FasterStereoCuda.h:39-51), same fields and defaults."""
+    width: int = 0
+    height: int = 0
+    min_disp: int = 0
+    max_disp: int = 0
+    num_layers: int = 1
+    do_lr_check: bool = True
+    do_rm_peaks: bool = True
+    do_smooth: bool = False
+
+
+def reference_schedule(opt: StereoOption1) -> CuSGMOption:
+    """Options FasterStereoCuda.cpp:169-199 gives every level (the level
+    dependent parts -- uniqueness, LR level, dilation -- are applied by the
+    pyramid itself)."""
+    return CuSGMOption(p1=8, p2_init=32, p2_max=64, cs_type=CuSGMOption.CS_5x5, num_paths=4,
+                       do_median_filter=True, do_lr_check=opt.do_lr_check, lr_threshold=1.0,
+                       unique_threshold=0.95, using_constant_p2=True,
+                       do_remove_peaks=opt.do_rm_peaks, peaks_ratio_threshold=0.0005,
+                       post_filter_type=1 if opt.do_smooth else 0)
+
+
+class FasterStereoCuda:
+    LEVEL_RANGE = 32
+
+    def __init__(self):
+        self._pyr = None
+        self.last_error = ""
+
+    def Init(self, option: StereoOption1) -> bool:
+        if option.width <= 0 or option.height <= 0:
+            self.last_error = "image size error"
+            return False
+        try:
+            self._pyr = SgmPyramid(option.width, option.height, option.min_disp, option.max_disp,
+                                   max(1, option.num_layers), self.LEVEL_RANGE,
+                                   reference_schedule(option))
+        except RuntimeError as e:
+            self.last_error = str(e)
+            self._pyr = None
+            return False
+        return True
+
+    def Match(self, bytes_left: np.ndarray, bytes_right: np.ndarray, disparity: np.ndarray) -> bool:
+        if self._pyr is None:
+            self.last_error = "not initialised"
+            return False
+        try:
+            disparity[...] = self._pyr.match(bytes_left, bytes_right)
+        except _lib.SgmError as e:
+            self.last_error = str(e)
+            return False
+        return True
+
+    def Release(self) -> None:
+        self._pyr = None

@@ -74,3 +74,50 @@ def post(raw_left: torch.Tensor, raw_right: torch.Tensor | None, do_lr: bool, lr
                                W, H, int(do_lr), lr_threshold, int(do_median), out_l.data_ptr(),
                                None if out_r is None else out_r.data_ptr(), _stream()))
     return out_l, out_r
+
+
+# ---- coarse-to-fine pyramid stages -------------------------------------
+def zoom_half(img: torch.Tensor) -> torch.Tensor:
+    _dev(img, torch.uint8)
+    h, w = img.shape
+    out = torch.empty((h // 2, w // 2), dtype=torch.uint8, device=img.device)
+    check(lib().sgm_stage_zoom(img.data_ptr(), w, h, out.data_ptr(), _stream()))
+    return out
+
+
+def init_next(coarse: torch.Tensor, w: int, h: int, level_range: int) -> torch.Tensor:
+    _dev(coarse, torch.float32)
+    out = torch.empty((h, w), dtype=torch.int16, device=coarse.device)
+    check(lib().sgm_stage_init_next(coarse.data_ptr(), w, h, level_range, out.data_ptr(), _stream()))
+    return out
+
+
+def cost_init(census_l, census_r, init, level_range: int, cs_type: int = 1) -> torch.Tensor:
+    _dev(census_l, torch.int64)
+    _dev(census_r, torch.int64)
+    _dev(init, torch.int16)
+    h, w = census_l.shape
+    out = torch.empty((h, w, level_range), dtype=torch.uint8, device=census_l.device)
+    check(lib().sgm_stage_cost_init(census_l.data_ptr(), census_r.data_ptr(), init.data_ptr(), w, h,
+                                    level_range, cs_type, out.data_ptr(), _stream()))
+    return out
+
+
+def aggregate_init(cost_vol, init, p1: int, p2: int):
+    _dev(cost_vol, torch.uint8)
+    _dev(init, torch.int16)
+    h, w, lr = cost_vol.shape
+    paths = [torch.zeros_like(cost_vol) for _ in range(4)]
+    ptrs = PP4(*[p.data_ptr() for p in paths])
+    check(lib().sgm_stage_aggregate_init(cost_vol.data_ptr(), init.data_ptr(), w, h, lr, p1, p2, ptrs,
+                                         _stream()))
+    return paths
+
+
+def wta_init(paths, init, uniqueness: float) -> torch.Tensor:
+    h, w, lr = paths[0].shape
+    out = torch.empty((h, w), dtype=torch.float32, device=paths[0].device)
+    ptrs = PP4(*[p.data_ptr() for p in paths])
+    check(lib().sgm_stage_wta_init(ptrs, _dev(init, torch.int16).data_ptr(), w, h, lr, uniqueness,
+                                   out.data_ptr(), _stream()))
+    return out

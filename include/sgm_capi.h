@@ -158,6 +158,27 @@ int sgm_dsgm_backward(const float* d_cost, const float* d_p1, const float* d_p2,
                       float temperature, float* d_grad_cost, float* d_grad_p1,
                       float* d_grad_p2, float* d_work, void* stream);
 
+/* ---- coarse-to-fine pyramid ---------------------------------------------
+ * Replaces HierSgmCudaImpl (FasterStereoCuda/FasterStereoCuda.cpp:203-295
+ * Init, :408-529 Match) with every level on the GPU.  Level i is
+ * (W >> i) x (H >> i); the coarsest level runs the plain pipeline over
+ * [min/2^i, max/2^i) rounded up to 2^k >= 64 (LR check there if
+ * opt->do_lr_check); each finer level searches level_range (16 or 32)
+ * disparities around 2 d_coarse - level_range/2.  opt->unique_threshold
+ * applies at level 0, 1.0 above.  NULL on failure (sgm_last_error(NULL)). */
+typedef struct sgm_pyramid sgm_pyramid;
+sgm_pyramid* sgm_pyramid_create(int32_t width, int32_t height, int32_t min_disp, int32_t max_disp,
+                                int32_t num_layers, int32_t level_range, const sgm_option* opt);
+/* host pointers, blocking */
+int sgm_pyramid_match(sgm_pyramid* p, const uint8_t* left, const uint8_t* right, float* disp);
+/* device pointers, enqueued on stream */
+int sgm_pyramid_match_device(sgm_pyramid* p, const uint8_t* d_left, const uint8_t* d_right,
+                             float* d_disp, void* stream);
+/* range of the coarsest level (and its minimum disparity through min_disp) */
+int32_t sgm_pyramid_coarsest_range(const sgm_pyramid* p, int32_t* min_disp);
+size_t sgm_pyramid_device_bytes(const sgm_pyramid* p);
+void sgm_pyramid_destroy(sgm_pyramid* p);
+
 /* ---- stage entry points (device pointers; used by the parity tests) ----
  * Each runs one stage of the plain pipeline over one view.  Layouts:
  * census u64 [H][W]; cost and path volumes u8 [H][W][D] (d fastest). */
@@ -186,6 +207,24 @@ int sgm_stage_wta(uint8_t* const d_paths[4], int32_t width, int32_t height,
 int sgm_stage_post(const float* d_raw_left, const float* d_raw_right, int32_t width,
                    int32_t height, int32_t do_lr_check, float lr_threshold,
                    int32_t do_median, float* d_out_left, float* d_out_right, void* stream);
+
+/* FasterStereoCuda.cpp:114-136: d_dst[(h/2)][(w/2)] = d_src[2i][2j]. */
+int sgm_stage_zoom(const uint8_t* d_src, int32_t w, int32_t h, uint8_t* d_dst, void* stream);
+/* FasterStereoCuda.cpp:138-166: base disparities of a w x h level from the
+ * (w/2) x (h/2) coarse map. */
+int sgm_stage_init_next(const float* d_coarse, int32_t w, int32_t h, int32_t level_range,
+                        int16_t* d_next, void* stream);
+/* cusgm_cc.cu:173-237 / 294-360: window cost [h][w][level_range]. */
+int sgm_stage_cost_init(const uint64_t* d_census_l, const uint64_t* d_census_r,
+                        const int16_t* d_init, int32_t w, int32_t h, int32_t level_range,
+                        int32_t cs_type, uint8_t* d_cost, void* stream);
+/* cusgm_ca.cu:1181-1339: window aggregation of the 4 directions. */
+int sgm_stage_aggregate_init(const uint8_t* d_cost, const int16_t* d_init, int32_t w, int32_t h,
+                             int32_t level_range, int32_t p1, int32_t p2,
+                             uint8_t* const d_paths[4], void* stream);
+/* cusgm_ca.cu:3425-3512: window WTA. */
+int sgm_stage_wta_init(uint8_t* const d_paths[4], const int16_t* d_init, int32_t w, int32_t h,
+                       int32_t level_range, float uniqueness, float* d_disp, void* stream);
 
 #ifdef __cplusplus
 }

@@ -230,6 +230,9 @@ def _plib():
         L.orc_match_init.restype = I
         L.orc_pyramid.argtypes = [P, P, I, I, I, I, I, I, ctypes.POINTER(Options), P]
         L.orc_pyramid.restype = I
+        L.orc_dilation.argtypes = [P, P, I, I, I]
+        L.orc_dilation_window.argtypes = [I, I]
+        L.orc_dilation_window.restype = I
         L._pyr_ready = True
     return L
 
@@ -271,6 +274,18 @@ def wta_init(paths, init, uniqueness):
     _plib().orc_wta_init(*[_p(p) for p in paths], _p(np.ascontiguousarray(init, np.int16)), W, H,
                          D, uniqueness, _p(out))
     return out
+
+
+def dilation(disp, wnd):
+    disp = np.ascontiguousarray(disp, np.float32)
+    H, W = disp.shape
+    out = np.empty_like(disp)
+    _plib().orc_dilation(_p(disp), _p(out), W, H, wnd)
+    return out
+
+
+def dilation_window(w, h):
+    return _plib().orc_dilation_window(w, h)
 
 
 def match_init(left, right, init, level_range, cs_type=1, p1=10, p2=150, uniqueness=0.95,

@@ -470,6 +470,29 @@ void orc_wta_init(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2, const
     }
 }
 
+/* cusgm_df.cu:188-218 (Kernel_Dilation, float): an INVALID pixel strictly
+ * inside the border (x > r, x < W - r, y > r, y < H - r, r = (wnd-1)/2)
+ * takes the maximum of its wnd x wnd window; other pixels are copied.
+ * The pyramid applies it to the coarsest level with wnd = 3 / 5 / 7 for
+ * min(w, h) < 360 / < 720 / otherwise (cusgm_df.cu:300-304). */
+void orc_dilation(const float* in, float* out, int W, int H, int wnd) {
+    const int r = (wnd - 1) / 2;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float v = in[(size_t)y * W + x];
+            if (v == ORC_INVALID_DISP && x > r && x < W - r && y > r && y < H - r)
+                for (int i = -r; i <= r; ++i)
+                    for (int j = -r; j <= r; ++j) v = std::max(v, in[(size_t)(y + j) * W + x + i]);
+            out[(size_t)y * W + x] = v;
+        }
+}
+
+int orc_dilation_window(int w, int h) {
+    const int s = std::min(w, h);
+    return s < 360 ? 3 : (s < 720 ? 5 : 7);
+}
+
 int orc_match_init(const uint8_t* left, const uint8_t* right, const int16_t* init, int W, int H,
                    int level_range, const orc_options* opt, float* disp) {
     if (!left || !right || !init || !disp || W <= 0 || H <= 0) return -1;
@@ -521,6 +544,11 @@ int orc_pyramid(const uint8_t* left, const uint8_t* right, int W, int H, int min
     if (top == 0) return orc_match(left, right, W, H, dmin, range, &o, disp, nullptr);
     int rc = orc_match(L[top].data(), R[top].data(), lw[top], lh[top], dmin, range, &o, cur.data(), nullptr);
     if (rc) return rc;
+    {   // the coarsest level of a multi-level pyramid is dilated (FasterStereoCuda.cpp:192-197)
+        std::vector<float> dil(cur.size());
+        orc_dilation(cur.data(), dil.data(), lw[top], lh[top], orc_dilation_window(lw[top], lh[top]));
+        cur.swap(dil);
+    }
     o.do_lr_check = 0;
     for (int n = top - 1; n >= 0; --n) {
         std::vector<int16_t> init((size_t)lw[n] * lh[n]);

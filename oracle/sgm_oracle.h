@@ -76,10 +76,13 @@ void orc_aggregate_init(const uint8_t* cost, const int16_t* init, int W, int H,
 void orc_wta_init(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2, const uint8_t* p3,
                   const int16_t* init, int W, int H, int level_range, float uniqueness,
                   float* disp);
+void orc_dilation(const float* in, float* out, int W, int H, int wnd);
+int orc_dilation_window(int w, int h);
 /* One finer level: census, init cost, 4 paths, init WTA, [median]. */
 int orc_match_init(const uint8_t* left, const uint8_t* right, const int16_t* init, int W, int H,
                    int level_range, const orc_options* opt, float* disp);
-/* Whole pyramid: num_layers levels, coarsest plain (with LR if opt->do_lr_check),
+/* Whole pyramid: num_layers levels, coarsest plain (with LR if opt->do_lr_check,
+ * then a dilation when num_layers >= 2),
  * finer levels searched in a window of level_range around the propagated
  * disparity; uniqueness opt->uniqueness at level 0 and 1.0 above. */
 int orc_pyramid(const uint8_t* left, const uint8_t* right, int W, int H, int min_disp,

@@ -1,0 +1,105 @@
+"""Coarse-to-fine pyramid on the GPU against the CPU oracle: every window
+stage bit-exact, then whole pyramids (window 16 and 32) including the
+4800x3409, D = 256, 3-level workload."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from fasterstereocuda_amd import scenes, stages
+from fasterstereocuda_amd.pyramid import FasterStereoCuda, SgmPyramid, StereoOption1
+from fasterstereocuda_amd.stereo import CuSGMOption
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def _random_init(rng, h, w, lr, invalid_frac=0.15):
+    init = rng.integers(-3, 50, size=(h, w)).astype(np.int16)
+    # piecewise-constant runs with jumps, like propagated disparities
+    init = np.repeat(init[:, ::4], 4, axis=1)[:, :w].copy()
+    init[rng.random((h, w)) < invalid_frac] = oracle.INVALID_SHORT
+    return init
+
+
+@pytest.mark.parametrize("lr", [16, 32])
+@pytest.mark.parametrize("cs", [1, 0])
+def test_window_stages_match_oracle(cuda, lr, cs):
+    print(f"[window stages] lr={lr} cs={cs}", flush=True)
+    rng = np.random.default_rng(lr + cs)
+    pair = scenes.cone_like(2, H=57, W=131, D=64)
+    h, w = pair.left.shape
+    init = _random_init(rng, h, w, lr)
+    cl = oracle.census(pair.left, cs, False)
+    cr = oracle.census(pair.right, cs, True)
+    ref_cost = oracle.cost_init(cl, cr, init, lr, cs)
+    g_init = torch.from_numpy(init).to(cuda)
+    g_cost = stages.cost_init(torch.from_numpy(cl.view(np.int64)).to(cuda),
+                              torch.from_numpy(cr.view(np.int64)).to(cuda), g_init, lr, cs)
+    np.testing.assert_array_equal(g_cost.cpu().numpy(), ref_cost)
+    p1, p2 = (10, 150) if cs else (8, 32)
+    paths = stages.aggregate_init(g_cost, g_init, p1, p2)
+    ref_paths = [oracle.aggregate_init(ref_cost, init, r, p1, p2) for r in range(4)]
+    for r in range(4):
+        np.testing.assert_array_equal(paths[r].cpu().numpy(), ref_paths[r], err_msg=f"direction {r}")
+    for uniq in (0.95, 1.0):
+        d = stages.wta_init(paths, g_init, uniq)
+        np.testing.assert_array_equal(_bits(d.cpu().numpy()), _bits(oracle.wta_init(ref_paths, init, uniq)))
+
+
+@pytest.mark.parametrize("w,h", [(130, 57), (131, 58), (64, 33)])
+def test_zoom_and_init_propagation(cuda, w, h):
+    rng = np.random.default_rng(w * h)
+    img = rng.integers(0, 256, size=(h, w), dtype=np.uint8)
+    z = stages.zoom_half(torch.from_numpy(img).to(cuda))
+    np.testing.assert_array_equal(z.cpu().numpy(), oracle.zoom_half(img))
+    coarse = rng.uniform(0, 30, size=(h // 2, w // 2)).astype(np.float32)
+    coarse[rng.random(coarse.shape) < 0.2] = oracle.INVALID
+    for lr in (16, 32):
+        nxt = stages.init_next(torch.from_numpy(coarse).to(cuda), w, h, lr)
+        np.testing.assert_array_equal(nxt.cpu().numpy(), oracle.init_next(coarse, w, h, lr))
+
+
+@pytest.mark.parametrize("W,H,D,layers,lr,cs", [(400, 300, 128, 3, 16, 1), (400, 300, 128, 3, 32, 1),
+                                                (333, 250, 64, 2, 32, 0), (256, 200, 64, 1, 32, 1)])
+def test_pyramid_matches_oracle(cuda, W, H, D, layers, lr, cs):
+    print(f"[pyramid] {W}x{H} D{D} layers{layers} lr{lr} cs{cs}", flush=True)
+    p = scenes.cone_like(5, H=H, W=W, D=D, workload=2)
+    opt = CuSGMOption.benchmark(cs_type=cs)
+    pyr = SgmPyramid(W, H, 0, D, layers, lr, opt)
+    got = pyr.match(p.left, p.right)
+    exp = oracle.pyramid(p.left, p.right, 0, D, layers, lr, cs_type=cs)
+    np.testing.assert_array_equal(_bits(got), _bits(exp))
+    again = pyr.match(p.left, p.right)
+    np.testing.assert_array_equal(_bits(again), _bits(got))
+
+
+def test_building_4800x3409_pyramid(cuda):
+    """The large-image workload: 4800x3409, D = 256, 3 levels, window 16."""
+    p = scenes.building(0)
+    print("[pyramid] building 4800x3409", flush=True)
+    pyr = SgmPyramid(4800, 3409, 0, 256, 3, 16, CuSGMOption.benchmark())
+    assert pyr.coarsest == (0, 64)
+    got = pyr.match(p.left, p.right)
+    exp = oracle.pyramid(p.left, p.right, 0, 256, 3, 16)
+    np.testing.assert_array_equal(_bits(got), _bits(exp))
+    valid = got != oracle.INVALID
+    assert valid.mean() > 0.5
+
+
+def test_faster_stereo_cuda_mirror(cuda):
+    p = scenes.cone_like(1, H=300, W=400, D=128, workload=2)
+    fs = FasterStereoCuda()
+    opt = StereoOption1(width=400, height=300, min_disp=0, max_disp=128, num_layers=2,
+                        do_lr_check=True, do_rm_peaks=False)
+    assert fs.Init(opt), fs.last_error
+    out = np.empty((300, 400), np.float32)
+    assert fs.Match(p.left, p.right, out)
+    exp = oracle.pyramid(p.left, p.right, 0, 128, 2, 32, cs_type=0, p1=8, p2=32)
+    np.testing.assert_array_equal(_bits(out), _bits(exp))
+    bad = FasterStereoCuda()
+    assert not bad.Init(StereoOption1(width=400, height=300, max_disp=128))   # rm-peaks default on
+    assert "remove-peaks" in bad.last_error

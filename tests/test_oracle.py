@@ -320,3 +320,133 @@ def test_pipeline_is_deterministic():
     b = oracle.match(p.left, p.right, 0, 64)
     np.testing.assert_array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
     np.testing.assert_array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+
+
+# ------------------------------------------------------------------ pyramid
+def test_zoom_half_numpy():
+    rng = np.random.default_rng(0)
+    img = rng.integers(0, 256, size=(37, 51), dtype=np.uint8)
+    np.testing.assert_array_equal(oracle.zoom_half(img), img[0:36:2, 0:50:2])
+
+
+def test_init_next_numpy():
+    rng = np.random.default_rng(1)
+    w, h, lr = 20, 15, 16
+    coarse = rng.uniform(-3, 40, size=(h // 2, w // 2)).astype(np.float32)
+    coarse[rng.random(coarse.shape) < 0.3] = INV
+    exp = np.empty((h, w), np.int16)
+    for i in range(h):
+        for j in range(w):
+            if i % 2 == 1 or i == h - 1:
+                exp[i, j] = exp[i - 1, j]
+            else:
+                d = coarse[i >> 1, j >> 1]
+                exp[i, j] = -32768 if d == INV else np.int16(np.trunc(np.float32(2) * d - np.float32(lr // 2)))
+    np.testing.assert_array_equal(oracle.init_next(coarse, w, h, lr), exp)
+
+
+@pytest.mark.parametrize("cs_type", [0, 1])
+@pytest.mark.parametrize("lr", [16, 32])
+def test_cost_init_constant_base_is_plain_cost_plus_adjust(cs_type, lr):
+    rng = np.random.default_rng(lr)
+    H, W, b = 6, 70, 9
+    cl = rng.integers(0, 2**62, size=(H, W), dtype=np.uint64)
+    cr = rng.integers(0, 2**62, size=(H, W), dtype=np.uint64)
+    init = np.full((H, W), b, np.int16)
+    got = oracle.cost_init(cl, cr, init, lr, cs_type)
+    plain = oracle.cost(cl, cr, b, 64, True, 1)            # C[x][t] = popc(cl[x] ^ cr[x - b - t])
+    adjust = np.floor(np.exp(4 * np.abs(np.arange(lr) - lr // 2) // (lr // 2))).astype(np.int64)
+    x = np.arange(W)[None, :, None]
+    t = np.arange(lr)[None, None, :]
+    inside = ((x - t - b) > 0) & ((x - t - b) < W)
+    exp = np.where(inside, np.zeros((H, W, lr), np.int64), 0xFF if cs_type else 127)
+    full = np.zeros((H, W, lr), np.int64)
+    full[:, b:, :] = plain[:, b:, :lr]
+    exp = np.where(inside, full + adjust[None, None, :], exp)
+    np.testing.assert_array_equal(got, exp.astype(np.uint8))
+
+
+@pytest.mark.parametrize("direction", [0, 1, 2, 3])
+def test_aggregate_init_constant_base_is_plain(direction):
+    """With the same base at every pixel the window never shifts and the P2
+    divisor is |b - b + 1| = 1: the recurrence is the plain one."""
+    rng = np.random.default_rng(direction)
+    cost = rng.integers(0, 120, size=(9, 11, 16), dtype=np.uint8)
+    init = np.full((9, 11), 5, np.int16)
+    got = oracle.aggregate_init(cost, init, direction, 10, 150)
+    np.testing.assert_array_equal(got, oracle.aggregate(cost, 0, direction, 10, 150))
+
+
+def test_aggregate_init_invalid_pixels_and_shift():
+    """Hand case on one row: an INVALID base sets m' = 127 and skips the
+    pixel; the next pixel's min(..) - m' then wraps as uint32."""
+    lr = 16
+    cost = np.zeros((1, 4, lr), np.uint8)
+    cost[0, 0] = np.arange(lr) + 1          # L0 = 1..16, m = 1
+    cost[0, 2] = 30
+    cost[0, 3] = 5
+    init = np.array([[3, -32768, 4, 4]], np.int16)
+    got = oracle.aggregate_init(cost, init, 1, 10, 150, fill=77)
+    np.testing.assert_array_equal(got[0, 0], np.arange(lr) + 1)
+    assert (got[0, 1] == 77).all()          # skipped pixel left untouched
+    # pixel 2: b = 4, b' = 3, m' = 127: candidates are L'(t), L'(t+2)+P1, 127 terms,
+    # 127 + max(10, 150 / |3 - 4 + 1| -> 150/1) ; result = 30 + (min - 127) mod 2^32 mod 256
+    Lp = np.arange(lr, dtype=np.int64) + 1
+    exp2 = []
+    for t in range(lr):
+        i1, i2 = t + 4 - 1 - 3, t + 4 + 1 - 3
+        bm = Lp[i1] + 10 if 0 <= i1 < lr else 127
+        bp = Lp[i2] + 10 if 0 <= i2 < lr else 127
+        v = min(Lp[t], bm, bp, 127 + 150)
+        exp2.append((30 + ((v - 127) % 2**32)) % 2**32)
+    np.testing.assert_array_equal(got[0, 2], np.array(exp2) % 256)
+
+
+def test_dilation_numpy():
+    rng = np.random.default_rng(3)
+    d = rng.uniform(0, 50, size=(20, 25)).astype(np.float32)
+    d[rng.random(d.shape) < 0.4] = INV
+    for wnd in (3, 5, 7):
+        r = (wnd - 1) // 2
+        exp = d.copy()
+        for y in range(20):
+            for x in range(25):
+                if d[y, x] == INV and r < x < 25 - r and r < y < 20 - r:
+                    exp[y, x] = d[y - r:y + r + 1, x - r:x + r + 1].max()
+        np.testing.assert_array_equal(oracle.dilation(d, wnd), exp)
+    assert [oracle.dilation_window(w, h) for w, h in ((400, 300), (1200, 852), (1200, 700))] == [3, 7, 5]
+
+
+def test_wta_init_numpy():
+    rng = np.random.default_rng(6)
+    H, W, lr = 14, 13, 16
+    paths = [rng.integers(0, 60, size=(H, W, lr), dtype=np.uint8) for _ in range(4)]
+    init = rng.integers(0, 30, size=(H, W)).astype(np.int16)
+    init[3, 7] = -32768
+    got = oracle.wta_init(paths, init, 0.95)
+    S = sum(p.astype(np.int64) for p in paths)
+    exp = np.full((H, W), INV, np.float32)
+    ut = np.float32(1) - np.float32(0.95)
+    for y in range(4 * (H // 4)):
+        for x in range(W):
+            if x > W - 5 or y > H - 5 or x < 5 or y < 5 or init[y, x] == -32768:
+                continue
+            s = S[y, x]
+            ds = int(np.argmin(s))
+            if ds <= 1 or ds >= lr - 1:
+                continue
+            c1, c2, c3 = np.float32(s[ds - 1]), np.float32(s[ds]), np.float32(s[ds + 1])
+            if min(c1 - c2, c3 - c2) <= c2 * ut:
+                continue
+            den = max(np.float32(1), c1 + c3 - np.float32(2) * c2)
+            exp[y, x] = np.float32(ds + int(init[y, x])) + (c1 - c3 + den) / (np.float32(2) * den)
+    np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+def test_pyramid_recovers_ground_truth():
+    p = scenes.building(0, H=852, W=1200, D=256)
+    d = oracle.pyramid(p.left, p.right, 0, 256, 3, 16)
+    valid = d != INV
+    assert valid.mean() > 0.6
+    err = np.abs(d[valid] - 0.5 - p.disparity[valid])
+    assert (err > 2).mean() < 0.1
