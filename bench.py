@@ -9,9 +9,11 @@ outputs resident in HBM.  The headline value is the batched path
 
 Prints ONE JSON line:
   value          W*H*D/s summed over the batch (batched path)
-  roofline       aggregation kernel: algorithmic 6 B/cell/view (4 B cost
-                 reads + 2 B summed output) / its average duration from HIP
-                 events recorded around the launch during the timed steps
+  roofline       aggregation + WTA (two launches: three directions, then
+                 bottom->top fused with the WTA): algorithmic 8 B/cell/view
+                 (4 B cost reads + 2 B summed output + 2 B read by the WTA) /
+                 their average duration from HIP events recorded around the
+                 launches during the timed steps
   cpu_baseline   the OpenMP oracle on a bounded sample of the same pairs
 """
 from __future__ import annotations
@@ -26,7 +28,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-AGG_BYTES_PER_CELL = 6.0     # per view: 4 path reads of the u8 cost + one u16 sum write
+# Algorithmic bytes per cell and view of aggregation + WTA: 4 direction reads
+# of the u8 cost + one u16 4-direction sum write (6 B), and the WTA's read of
+# that sum (2 B).  The engine runs them as two launches: three directions,
+# then bottom->top fused with the WTA.
+AGG_BYTES_PER_CELL = 8.0
 
 
 def parse():
@@ -43,6 +49,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a float32 negate of 256 MiB (counter calibration)")
+    ap.add_argument("--workload", default="driving",
+                    choices=["driving", "building", "uneven", "diffsgm"],
+                    help="driving: the default 32-pair batch; the others measure the "
+                         "large-image pyramid, the uneven batch and the differentiable form")
     return ap.parse_args()
 
 
@@ -54,6 +64,10 @@ def main():
               f"WORLD_SIZE={world}); pairs are independent, so several GPUs would each run "
               "this same single-GPU batch on their own share of the pairs.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "driving":
+        run_other(args)
+        return
 
     import numpy as np
     import torch
@@ -98,7 +112,7 @@ def main():
     value = NP * cells_per_pair * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
     stage_avg = {k: v / max(runs, 1) for k, v in stage_ms.items()}
-    agg_ms = stage_avg["aggregate"]
+    agg_ms = stage_avg["aggregate3"] + stage_avg["aggregate_wta"]
     agg_bytes = AGG_BYTES_PER_CELL * cells_per_pair * NP * 2      # both views
     achieved = agg_bytes / (agg_ms * 1e-3) / 1e9
 
@@ -190,7 +204,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "aggregate_kernel",
+            "kernel": "aggregate_kernel<1> + aggregate_kernel<2> (bottom->top fused with WTA)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -205,6 +219,190 @@ def main():
         "stage_ms": stage_avg,
     }
     print(json.dumps(line), flush=True)
+
+
+# ---------------------------------------------------------------- other workloads
+# building: 4800x3409, D = 256 -- 3-level pyramid (window 16) and the dense
+#           single-level D = 256 pipeline (a ~4.2 GB cost volume per view)
+# uneven:   64 pairs, five formats, D in {64, 128} -- batched vs plain
+# diffsgm:  differentiable FP32 SGM forward + backward, batch 8, 1242x375, D = 128
+
+def _emit(line):
+    print(json.dumps(line), flush=True)
+
+
+def _common(args, metric, value, unit, ms, hib, config, **extra):
+    line = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": hib, "scaling": "weak",
+            "vs_baseline": None, "data": "synthetic", "config": config}
+    line.update(extra)
+    return line
+
+
+def building(args):
+    import numpy as np
+    import torch
+    from fasterstereocuda_amd import scenes
+    from fasterstereocuda_amd.pyramid import SgmPyramid
+    from fasterstereocuda_amd.stereo import CuSGMOption, StereoCuda
+
+    W, H, D = 4800, 3409, 256
+    p = scenes.building(0)
+    dev = torch.device("cuda:0")
+    L = torch.from_numpy(p.left).to(dev)
+    R = torch.from_numpy(p.right).to(dev)
+    out = torch.empty((H, W), dtype=torch.float32, device=dev)
+    opt = CuSGMOption.benchmark()
+    pyr = SgmPyramid(W, H, 0, D, 3, 16, opt)
+    for _ in range(args.warmup):
+        pyr.match_device(L, R, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pyr.match_device(L, R, out)
+    torch.cuda.synchronize()
+    pyr_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    pyr_out = out.cpu().numpy()
+    # dense single level, D = 256 (both views, LR check): streams ~4.2 GB per volume
+    st = StereoCuda()
+    assert st.Init(W, H, 0, D, opt), st.last_error
+    dense = torch.empty_like(out)
+    st.match_device(L, R, dense)
+    torch.cuda.synchronize()
+    dsteps = max(1, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(dsteps):
+        st.match_device(L, R, dense)
+    torch.cuda.synchronize()
+    dense_ms = (time.perf_counter() - t0) * 1e3 / dsteps
+    st.Release()
+    cells = W * H * D
+    valid = pyr_out != -999999.0
+    _emit(_common(args, "W*H*D/s (nominal D) large-image pyramid", cells / (pyr_ms * 1e-3), "W*H*D/s",
+                  pyr_ms, True,
+                  {"workload": "building 4800x3409, D=256, 3-level pyramid, window 16, 9x7 census, "
+                               "P1=10 P2=150, LR at the coarsest level, median, dilation",
+                   "width": W, "height": H, "disp_range": D},
+                  dtype="u8",
+                  dense_single_level={"ms": dense_ms, "value": cells / (dense_ms * 1e-3),
+                                      "unit": "W*H*D/s"},
+                  pyramid_valid_fraction=float(valid.mean())))
+
+
+def uneven(args):
+    import numpy as np
+    import torch
+    from fasterstereocuda_amd import scenes
+    from fasterstereocuda_amd.batch import SgmBatch
+    from fasterstereocuda_amd.stereo import CuSGMOption, StereoCuda
+
+    dev = torch.device("cuda:0")
+    opt = CuSGMOption.benchmark()
+    specs = scenes.uneven_specs(64)
+    pairs, cells = [], 0
+    for i, (w, h, d) in enumerate(specs):
+        sc = scenes.cone_like(i, H=h, W=w, D=d, workload=3)
+        pairs.append({"left": torch.from_numpy(sc.left).to(dev), "right": torch.from_numpy(sc.right).to(dev),
+                      "disp_left": torch.empty((h, w), dtype=torch.float32, device=dev),
+                      "min_disparity": 0, "disp_range": d})
+        cells += w * h * d
+    batch = SgmBatch(pairs, opt)
+    if getattr(args, "graph", True) and hasattr(batch, "capture"):
+        batch.capture()
+    for _ in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    torch.cuda.synchronize()
+    b_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # plain path: one handle per (W, H, D), one Match per pair
+    handles = {}
+    outs = []
+    for p in pairs:
+        h, w = p["left"].shape
+        key = (w, h, p["disp_range"])
+        if key not in handles:
+            st = StereoCuda()
+            assert st.Init(w, h, 0, p["disp_range"], opt), st.last_error
+            handles[key] = st
+        outs.append(torch.empty((h, w), dtype=torch.float32, device=dev))
+    def plain_pass():
+        for p, o in zip(pairs, outs):
+            h, w = p["left"].shape
+            handles[(w, h, p["disp_range"])].match_device(p["left"], p["right"], o)
+    plain_pass()
+    torch.cuda.synchronize()
+    psteps = max(1, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(psteps):
+        plain_pass()
+    torch.cuda.synchronize()
+    p_ms = (time.perf_counter() - t0) * 1e3 / psteps
+    same = all(bool((o.cpu().numpy().view(np.uint32) == p["disp_left"].cpu().numpy().view(np.uint32)).all())
+               for o, p in zip(outs, pairs))
+    _emit(_common(args, "W*H*D/s summed over an uneven batch (batched vs plain)", cells / (b_ms * 1e-3),
+                  "W*H*D/s", b_ms, True,
+                  {"workload": "64 pairs, sizes from {450x375, 640x480, 1242x375, 1280x720, 1920x1080}, "
+                               "D in {64, 128}, 9x7 census, P1=10 P2=150, uniqueness 0.95, LR, median",
+                   "pairs": 64, "path": "batched"},
+                  dtype="u8", plain_value=cells / (p_ms * 1e-3), plain_ms=p_ms,
+                  batched_over_plain=p_ms / b_ms, batched_bit_exact_vs_plain=same,
+                  device_bytes=batch.device_bytes))
+
+
+def diffsgm(args):
+    import numpy as np
+    import torch
+    from fasterstereocuda_amd import scenes
+    from fasterstereocuda_amd.diff_sgm import census_cost_volume, diff_sgm
+
+    dev = torch.device("cuda:0")
+    B, H, W, D = 8, 375, 1242, 128
+    sc = scenes.driving_batch(B, D=D, H=H, W=W, workload=4)
+    left = torch.from_numpy(np.stack([s.left for s in sc])).to(dev)
+    right = torch.from_numpy(np.stack([s.right for s in sc])).to(dev)
+    gt = torch.from_numpy(np.stack([s.disparity for s in sc])).to(dev)
+    cost = census_cost_volume(left, right, D)
+    rng = np.random.default_rng(4000)
+    p1 = torch.from_numpy(rng.uniform(0.05, 0.2, (B, H, W)).astype(np.float32)).to(dev)
+    p2 = torch.from_numpy(rng.uniform(0.5, 2.0, (B, H, W)).astype(np.float32)).to(dev)
+
+    def step():
+        c = cost.detach().requires_grad_(True)
+        a = p1.detach().requires_grad_(True)
+        b = p2.detach().requires_grad_(True)
+        disp = diff_sgm(c, a, b, 0.1)
+        loss = (disp - gt).abs().mean()
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    cells = B * H * W * D
+    bytes_alg = 50.0 * cells                      # ~24 B fwd + ~26 B bwd per cell
+    _emit(_common(args, "diff-SGM fwd+bwd ms/pair", ms / B, "ms/pair", ms, False,
+                  {"workload": "differentiable FP32 SGM, batch 8, 1242x375, D=128, per-pixel P1/P2, "
+                               "soft-argmin T=0.1, L1 loss", "batch": B, "width": W, "height": H,
+                   "disp_range": D},
+                  dtype="fp32", loss=float(loss),
+                  roofline={"bound": "hbm", "achieved": bytes_alg / (ms * 1e-3) / 1e9,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": bytes_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                            "traffic": None, "algorithmic_bytes_per_step": bytes_alg}))
+
+
+def run_other(args):
+    import torch
+    torch.cuda.set_device(0)
+    {"building": building, "uneven": uneven, "diffsgm": diffsgm}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -16,7 +16,7 @@ import ctypes
 from ._lib import SgmPairDesc, check, last_error, lib
 from .stereo import CuSGMOption
 
-STAGES = ("census", "cost", "aggregate", "wta", "post")
+STAGES = ("census", "cost", "aggregate3", "aggregate_wta", "post")
 
 
 class SgmBatch:

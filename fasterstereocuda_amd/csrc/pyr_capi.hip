@@ -184,6 +184,7 @@ sgm_pyramid* sgm_pyramid_create(int32_t width, int32_t height, int32_t min_disp,
         }
         post.out_l = p->out[top];
         post.W = w; post.H = h;
+        post.roi_x = std::max(0, dmin);
         post.do_lr = lr ? 1 : 0;
         post.do_median = opt->do_median_filter ? 1 : 0;
         post.lr_thr = opt->lr_threshold;
@@ -194,12 +195,7 @@ sgm_pyramid* sgm_pyramid_create(int32_t width, int32_t height, int32_t min_disp,
             free_pyramid(p);
             return nullptr;
         }
-        if (!lr) {
-            p->top_plan.njobs_ctw = 1;
-            p->top_plan.n_cost = cost_items(jobs[0]);
-            p->top_plan.n_agg = agg_items(jobs[0]);
-            p->top_plan.n_wta = wta_items(jobs[0]);
-        }
+        if (!lr) restrict_matching(&p->top_plan, jobs, 1);
     }
     // finer levels: census jobs + median post per level, window buffers
     for (int i = 0; i < top; ++i) {
@@ -237,13 +233,7 @@ int run_pyramid(sgm_pyramid* p, hipStream_t s) {
         PyrZoom z{p->il[i - 1], p->ir[i - 1], p->il[i], p->ir[i], p->lw[i - 1], p->lh[i - 1]};
         PYR_TRY(pyr_launch_zoom(z, s));
     }
-    StagePlan ctw = p->top_plan;
-    if (ctw.njobs_ctw > 0) ctw.njobs = ctw.njobs_ctw;
-    PYR_TRY(launch_census(p->top_plan, s));
-    PYR_TRY(launch_cost(ctw, s));
-    PYR_TRY(launch_aggregate(ctw, s));
-    PYR_TRY(launch_wta(ctw, s));
-    PYR_TRY(launch_post(p->top_plan, s));
+    PYR_TRY(launch_match(p->top_plan, false, s, nullptr));
     if (top > 0) {
         // the coarsest level of a multi-level pyramid is dilated
         // (FasterStereoCuda.cpp:192-197, cusgm_df.cu:300-318)

@@ -73,12 +73,15 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
                 StagePlan* plan, void** block, std::string* err) {
     const int nj = (int)jobs.size(), np = (int)posts.size();
     std::vector<int> oc(nj + 1, 0), ok(nj + 1, 0), oa(nj + 1, 0), ow(nj + 1, 0), op(np + 1, 0);
+    std::vector<int> o3(nj + 1, 0), od(nj + 1, 0);
     int dmax = 64;
     for (int i = 0; i < nj; ++i) {
         oc[i + 1] = oc[i] + census_items(jobs[i]);
         ok[i + 1] = ok[i] + cost_items(jobs[i]);
         oa[i + 1] = oa[i] + agg_items(jobs[i]);
         ow[i + 1] = ow[i] + wta_items(jobs[i]);
+        o3[i + 1] = o3[i] + agg3_items(jobs[i]);
+        od[i + 1] = od[i] + dw_items(jobs[i]);
         dmax = std::max(dmax, jobs[i].D);
     }
     for (int i = 0; i < np; ++i) op[i + 1] = op[i] + post_items(posts[i]);
@@ -86,7 +89,7 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     const size_t s_posts = align256(sizeof(SgmPost) * std::max(np, 1));
     const size_t s_oj = align256(sizeof(int) * (nj + 1));
     const size_t s_op = align256(sizeof(int) * (np + 1));
-    const size_t total = s_jobs + s_posts + 4 * s_oj + s_op;
+    const size_t total = s_jobs + s_posts + 6 * s_oj + s_op;
     std::vector<uint8_t> host(total, 0);
     size_t off = 0;
     auto put = [&](const void* src, size_t n, size_t slot) {
@@ -101,6 +104,8 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     const size_t a_ok = put(ok.data(), sizeof(int) * (nj + 1), s_oj);
     const size_t a_oa = put(oa.data(), sizeof(int) * (nj + 1), s_oj);
     const size_t a_ow = put(ow.data(), sizeof(int) * (nj + 1), s_oj);
+    const size_t a_o3 = put(o3.data(), sizeof(int) * (nj + 1), s_oj);
+    const size_t a_od = put(od.data(), sizeof(int) * (nj + 1), s_oj);
     const size_t a_op = put(op.data(), sizeof(int) * (np + 1), s_op);
     if (*block) {
         (void)hipFree(*block);
@@ -115,6 +120,8 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     plan->offs_cost = reinterpret_cast<int*>(b + a_ok);
     plan->offs_agg = reinterpret_cast<int*>(b + a_oa);
     plan->offs_wta = reinterpret_cast<int*>(b + a_ow);
+    plan->offs_agg3 = reinterpret_cast<int*>(b + a_o3);
+    plan->offs_dw = reinterpret_cast<int*>(b + a_od);
     plan->offs_post = reinterpret_cast<int*>(b + a_op);
     plan->njobs = nj;
     plan->nposts = np;
@@ -122,9 +129,48 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     plan->n_cost = ok[nj];
     plan->n_agg = oa[nj];
     plan->n_wta = ow[nj];
+    plan->n_agg3 = o3[nj];
+    plan->n_dw = od[nj];
     plan->n_post = op[np];
     plan->dmax = dmax;
     return SGM_OK;
+}
+
+// Runs cost .. WTA only for the first n jobs (the left views): their item
+// offsets come first, so find_job over n jobs stays valid.
+void fsgm::restrict_matching(StagePlan* P, const std::vector<SgmJob>& jobs, int n) {
+    P->njobs_ctw = n;
+    P->n_cost = P->n_agg = P->n_wta = P->n_agg3 = P->n_dw = 0;
+    for (int i = 0; i < n; ++i) {
+        P->n_cost += cost_items(jobs[i]);
+        P->n_agg += agg_items(jobs[i]);
+        P->n_wta += wta_items(jobs[i]);
+        P->n_agg3 += agg3_items(jobs[i]);
+        P->n_dw += dw_items(jobs[i]);
+    }
+}
+
+// census, cost, three directions, bottom->top + WTA, post, [right-map LR
+// invalidation]; marks[k] (may be null) are recorded before stage k and
+// after the last one.
+hipError_t fsgm::launch_match(const StagePlan& plan, bool right_invalidation, hipStream_t s,
+                              hipEvent_t* marks) {
+    StagePlan ctw = plan;
+    if (plan.njobs_ctw > 0) ctw.njobs = plan.njobs_ctw;
+    hipError_t e;
+    auto mark = [&](int k) { return marks ? hipEventRecord(marks[k], s) : hipSuccess; };
+    if ((e = mark(0)) != hipSuccess) return e;
+    if ((e = launch_census(plan, s)) != hipSuccess) return e;
+    if ((e = mark(1)) != hipSuccess) return e;
+    if ((e = launch_cost(ctw, s)) != hipSuccess) return e;
+    if ((e = mark(2)) != hipSuccess) return e;
+    if ((e = launch_aggregate3(ctw, s)) != hipSuccess) return e;
+    if ((e = mark(3)) != hipSuccess) return e;
+    if ((e = launch_aggregate_wta(ctw, s)) != hipSuccess) return e;
+    if ((e = mark(4)) != hipSuccess) return e;
+    if ((e = launch_post(plan, s)) != hipSuccess) return e;
+    if (right_invalidation && (e = launch_right_invalidate(plan, s)) != hipSuccess) return e;
+    return mark(5);
 }
 
 // Per-view device buffers of one pair.
@@ -201,44 +247,36 @@ int build_handle_plan(sgm_handle* h) {
     p.out_l = h->view[0].out;
     p.out_r = h->view[1].out;
     p.W = h->W; p.H = h->H;
+    p.roi_x = std::max(0, h->min_d);
     p.do_lr = h->opt.do_lr_check ? 1 : 0;
     p.do_median = h->opt.do_median_filter ? 1 : 0;
     p.lr_thr = h->opt.lr_threshold;
     std::vector<SgmPost> posts{p};
-    return upload_plan(jobs, posts, &h->plan, &h->plan_block, &h->err);
+    if (int rc = upload_plan(jobs, posts, &h->plan, &h->plan_block, &h->err)) return rc;
+    if (!h->opt.do_lr_check) restrict_matching(&h->plan, jobs, 1);
+    return SGM_OK;
 }
 
 // Runs the pipeline on the handle's own buffers.  With the LR check off the
 // right view is still censused (the left cost needs it) but not aggregated.
 int run_pipeline(sgm_handle* h, bool want_right, hipStream_t s) {
-    StagePlan full = h->plan;
-    StagePlan left_only = h->plan;
-    left_only.njobs = 1;
-    left_only.n_cost = cost_items(make_job(h, 0));
-    left_only.n_agg = agg_items(make_job(h, 0));
-    left_only.n_wta = wta_items(make_job(h, 0));
-    const StagePlan& ctw = h->opt.do_lr_check ? full : left_only;
-    using clk = std::chrono::steady_clock;
-    auto t0 = clk::now();
-    auto mark = [&](const char* what) {
-        if (!h->print_log) return;
-        (void)hipStreamSynchronize(s);
-        const auto t1 = clk::now();
-        std::printf("** %-24s %.3f ms\n", what,
-                    std::chrono::duration<double, std::milli>(t1 - t0).count());
-        t0 = t1;
-    };
-    HIP_TRY(&h->err, launch_census(full, s));
-    mark("Census:");
-    HIP_TRY(&h->err, launch_cost(ctw, s));
-    mark("Cost:");
-    HIP_TRY(&h->err, launch_aggregate(ctw, s));
-    mark("Aggregate:");
-    HIP_TRY(&h->err, launch_wta(ctw, s));
-    mark("WTA/SubPixel:");
-    HIP_TRY(&h->err, launch_post(full, s));
-    if (want_right && h->opt.do_lr_check) HIP_TRY(&h->err, launch_right_invalidate(full, s));
-    mark("LRCheck/Filter:");
+    const bool lr = h->opt.do_lr_check != 0;
+    if (!h->print_log) {
+        HIP_TRY(&h->err, launch_match(h->plan, want_right && lr, s, nullptr));
+        return SGM_OK;
+    }
+    hipEvent_t ev[6];
+    for (auto& e : ev) HIP_TRY(&h->err, hipEventCreate(&e));
+    HIP_TRY(&h->err, launch_match(h->plan, want_right && lr, s, ev));
+    HIP_TRY(&h->err, hipEventSynchronize(ev[5]));
+    static const char* names[5] = {"Census:", "Cost:", "Aggregate (3 paths):",
+                                   "Aggregate + WTA:", "LRCheck/Filter:"};
+    for (int k = 0; k < 5; ++k) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+        std::printf("** %-24s %.3f ms\n", names[k], ms);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
     return SGM_OK;
 }
 
@@ -517,6 +555,7 @@ sgm_batch* sgm_batch_create(const sgm_pair_desc* pairs, int32_t n, const sgm_opt
         post.out_l = p.d_disp_left;
         post.out_r = p.d_disp_right;
         post.W = p.width; post.H = p.height;
+        post.roi_x = std::max(0, p.min_disparity);
         post.do_lr = lr ? 1 : 0;
         post.do_median = opt->do_median_filter ? 1 : 0;
         post.lr_thr = opt->lr_threshold;
@@ -530,17 +569,7 @@ sgm_batch* sgm_batch_create(const sgm_pair_desc* pairs, int32_t n, const sgm_opt
         free_batch(b);
         return nullptr;
     }
-    if (!lr) {
-        // only the left jobs (the first n) go through cost .. wta
-        StagePlan& P = b->plan;
-        int nc = 0, na = 0, nw = 0;
-        for (int i = 0; i < n; ++i) {
-            nc += cost_items(jobs[i]); na += agg_items(jobs[i]); nw += wta_items(jobs[i]);
-        }
-        P.n_cost = nc; P.n_agg = na; P.n_wta = nw;
-        // find_job over the first n jobs still works: their offsets come first
-        b->plan.njobs_ctw = n;
-    }
+    if (!lr) restrict_matching(&b->plan, jobs, n);   // only the left views are matched
     return b;
 }
 
@@ -549,27 +578,12 @@ int sgm_batch_run(sgm_batch* b, void* stream) {
     HIP_TRY(&b->err, hipSetDevice(b->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t ev[6] = {};
-    if (b->timing)
+    if (b->timing) {
         for (int k = 0; k < 6; ++k) HIP_TRY(&b->err, take_event(b, &ev[k]));
-    auto mark = [&](int k) -> hipError_t {
-        if (!b->timing) return hipSuccess;
-        b->marks.push_back(ev[k]);
-        return hipEventRecord(ev[k], s);
-    };
-    StagePlan ctw = b->plan;
-    if (b->plan.njobs_ctw > 0) ctw.njobs = b->plan.njobs_ctw;
-    HIP_TRY(&b->err, mark(0));
-    HIP_TRY(&b->err, launch_census(b->plan, s));
-    HIP_TRY(&b->err, mark(1));
-    HIP_TRY(&b->err, launch_cost(ctw, s));
-    HIP_TRY(&b->err, mark(2));
-    HIP_TRY(&b->err, launch_aggregate(ctw, s));
-    HIP_TRY(&b->err, mark(3));
-    HIP_TRY(&b->err, launch_wta(ctw, s));
-    HIP_TRY(&b->err, mark(4));
-    HIP_TRY(&b->err, launch_post(b->plan, s));
-    if (b->want_right && b->opt.do_lr_check) HIP_TRY(&b->err, launch_right_invalidate(b->plan, s));
-    HIP_TRY(&b->err, mark(5));
+        b->marks.insert(b->marks.end(), ev, ev + 6);
+    }
+    HIP_TRY(&b->err, launch_match(b->plan, b->want_right && b->opt.do_lr_check, s,
+                                  b->timing ? ev : nullptr));
     return SGM_OK;
 }
 

@@ -9,6 +9,9 @@
 // pairs with different W, H and D.
 #pragma once
 #include <stdint.h>
+
+#include <vector>
+
 #include <hip/hip_runtime.h>
 
 namespace fsgm {
@@ -40,6 +43,7 @@ struct SgmPost {                   // one stereo pair: LR check + median
     float* out_l;
     float* out_r;                  // may be null
     int32_t W, H;
+    int32_t roi_x;                 // columns below are INVALID
     int32_t do_lr, do_median;
     float lr_thr;
 };
@@ -48,6 +52,8 @@ struct SgmPost {                   // one stereo pair: LR check + median
 inline int census_items(const SgmJob& j) { return ((j.W + 63) / 64) * ((j.H + 15) / 16); }
 inline int cost_items(const SgmJob& j) { return j.H * ((j.W - j.roi_x + 63) / 64); }
 inline int agg_items(const SgmJob& j) { return 2 * ((j.H + 3) / 4) + 2 * ((j.W - j.roi_x + 3) / 4); }
+inline int agg3_items(const SgmJob& j) { return 2 * ((j.H + 3) / 4) + ((j.W - j.roi_x + 3) / 4); }
+inline int dw_items(const SgmJob& j) { return (j.W - j.roi_x + 3) / 4; }
 inline int wta_items(const SgmJob& j) { return j.H * ((j.W + 15) / 16); }
 inline int post_items(const SgmPost& p) { return ((p.W + 15) / 16) * ((p.H + 15) / 16); }
 
@@ -58,21 +64,31 @@ struct StagePlan {
     int* offs_cost = nullptr;
     int* offs_agg = nullptr;
     int* offs_wta = nullptr;
+    int* offs_agg3 = nullptr;     // three directions
+    int* offs_dw = nullptr;       // bottom->top fused with the WTA
     SgmPost* posts = nullptr;     // device
     int* offs_post = nullptr;
     int njobs = 0, nposts = 0;
     int njobs_ctw = 0;            // if > 0: jobs [0, njobs_ctw) run cost..wta
-    int n_census = 0, n_cost = 0, n_agg = 0, n_wta = 0, n_post = 0;
+    int n_census = 0, n_cost = 0, n_agg = 0, n_wta = 0, n_post = 0, n_agg3 = 0, n_dw = 0;
     int dmax = 64;
 };
 
 // Launchers (sgm_kernels.hip).  All enqueue on `stream`.
 hipError_t launch_census(const StagePlan& p, hipStream_t s);
 hipError_t launch_cost(const StagePlan& p, hipStream_t s);
-hipError_t launch_aggregate(const StagePlan& p, hipStream_t s);
+hipError_t launch_aggregate(const StagePlan& p, hipStream_t s);        // 4 directions
+hipError_t launch_aggregate3(const StagePlan& p, hipStream_t s);       // LR, RL, UD
+hipError_t launch_aggregate_wta(const StagePlan& p, hipStream_t s);    // DU + WTA
 hipError_t launch_wta(const StagePlan& p, hipStream_t s);
 hipError_t launch_post(const StagePlan& p, hipStream_t s);
 hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s);
 hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t s);
+
+// sgm_capi.hip: the matching sequence shared by the plain, batched and
+// pyramid paths (marks: 6 events or null)
+void restrict_matching(StagePlan* P, const std::vector<SgmJob>& jobs, int n);
+hipError_t launch_match(const StagePlan& plan, bool right_invalidation, hipStream_t s,
+                        hipEvent_t* marks);
 
 }  // namespace fsgm

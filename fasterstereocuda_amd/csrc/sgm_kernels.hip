@@ -227,7 +227,12 @@ __global__ __launch_bounds__(256) void cost_kernel(const SgmJob* __restrict__ jo
 //   B-(d) = d == 0 ? 127 : L'(d-1) + P1,  B+(d) = d == D-1 ? 127 : L'(d+1) + P1
 //   m = min_d L(p, d); the stored value is L mod 256 (uint8 store).
 // All terms are < 2^16 when P1 + P2 <= 65000, so packed u16 is exact.
-template <int D>
+// With FUSE the direction is bottom->top and the scanline does the WTA of
+// each pixel it reaches instead of storing its own volume: S = the other three
+// stored directions + this one's value mod 256, argmin with the smallest d,
+// c1 / c3 gathered with one row OR-reduction, then the reference's
+// uniqueness and subpixel rules (see wta_pixel below).
+template <int D, bool FUSE>
 __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, int lane) {
     constexpr int DPD = D / 16;   // disparities per lane
     constexpr int NW = DPD / 4;   // u32 words of cost per lane
@@ -261,11 +266,70 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
     uint32_t L[NP];
 #pragma unroll
     for (int k = 0; k < NW; ++k) { L[2 * k] = bytes01(w[k]); L[2 * k + 1] = bytes23(w[k]); }
-    if (active) store_words<NW>(dst0, w);
+    if (active && !FUSE) store_words<NW>(dst0, w);
+    if (FUSE && active && l == 0) J.disp[y0 * W + x0] = kInvalidDisp;   // row H-1: border
     uint32_t t0 = L[0];
 #pragma unroll
     for (int k = 1; k < NP; ++k) t0 = pk_min(t0, L[k]);
     uint32_t m = row_min16(min(t0 & 0xFFFFu, t0 >> 16));
+
+    // WTA of the pixel at step i from this direction's L and the other three
+    // stored directions (FUSE only)
+    auto wta_step = [&](const uint32_t (&other)[3][NW], int i) __attribute__((always_inline)) {
+        uint32_t S[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const int wi = k >> 1;
+            uint32_t a = (k & 1) ? bytes23(other[0][wi]) : bytes01(other[0][wi]);
+            a = pk_add(a, (k & 1) ? bytes23(other[1][wi]) : bytes01(other[1][wi]));
+            a = pk_add(a, (k & 1) ? bytes23(other[2][wi]) : bytes01(other[2][wi]));
+            S[k] = pk_add(a, L[k] & 0x00FF00FFu);
+        }
+        uint32_t best = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const uint32_t d0 = (uint32_t)(l * DPD + 2 * k);
+            best = min(best, (S[k] << 16) | d0);
+            best = min(best, (S[k] & 0xFFFF0000u) | (d0 + 1));
+        }
+        best = row_min16(best);
+        const int ds = (int)(best & 0xFFFFu);
+        const uint32_t c2 = best >> 16;
+        // gather S(ds-1) and S(ds+1): one owner lane each, OR-reduced over the row
+        const int k1 = ds - 1 - l * DPD, k3 = ds + 1 - l * DPD;
+        uint32_t v1 = 0u, v3 = 0u;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            v1 = ((k1 >> 1) == k) ? S[k] : v1;
+            v3 = ((k3 >> 1) == k) ? S[k] : v3;
+        }
+        v1 = (k1 & 1) ? (v1 >> 16) : (v1 & 0xFFFFu);
+        v3 = (k3 & 1) ? (v3 >> 16) : (v3 & 0xFFFFu);
+        uint32_t both = ((k1 >= 0 && k1 < DPD) ? (v1 << 16) : 0u) | ((k3 >= 0 && k3 < DPD) ? v3 : 0u);
+        both |= dpp<0x128>(0u, both);
+        both |= dpp<0x124>(0u, both);
+        both |= dpp<0x122>(0u, both);
+        both |= dpp<0x121>(0u, both);
+        const int y = (int)y0 - i, x = (int)x0;
+        constexpr bool warp_rule = (D == 64 || D == 128);
+        const int rows_cov = warp_rule ? 2 * (H / 2) : H;
+        float outv = kInvalidDisp;
+        if (y < rows_cov && !(x > W - 5 || y > H - 5 || x < 5 || y < 5) && ds > 0 &&
+            (warp_rule || ds < D - 1)) {
+            const float c1 = (float)(both >> 16);
+            const float c2f = (float)c2;
+            const float c3 = (ds == D - 1) ? (float)(D - 1) : (float)(both & 0xFFFFu);
+            const float a = c1 - c2f, b = c3 - c2f;
+            const float uniq = (a < b) ? a : b;
+            if (!(uniq <= c2f * (1.0f - J.uniqueness))) {
+                const float t = c1 + c3 - 2.0f * c2f;
+                const float den = (1.0f < t) ? t : 1.0f;
+                const float bias = (float)((double)(c1 - c3 + den) / (double)(2.0f * den));
+                outv = (float)(ds + J.min_d) + bias;
+            }
+        }
+        if (active && l == 0) J.disp[(int64_t)y * W + x] = outv;
+    };
 
     // one recurrence step: consumes the cost words of step i, updates L, m
     auto advance = [&](const uint32_t (&c8)[NW], int i) __attribute__((always_inline)) {
@@ -291,39 +355,65 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
             L[k] = pk_sub(pk_add(c, v), mv);
             t = pk_min(t, L[k]);
         }
-        uint32_t o[NW];
+        if constexpr (!FUSE) {
+            uint32_t o[NW];
 #pragma unroll
-        for (int k = 0; k < NW; ++k) o[k] = pack4(L[2 * k], L[2 * k + 1]);
-        if (active) store_words<NW>(dst0 + (int64_t)i * step, o);
+            for (int k = 0; k < NW; ++k) o[k] = pack4(L[2 * k], L[2 * k + 1]);
+            if (active) store_words<NW>(dst0 + (int64_t)i * step, o);
+        }
         m = row_min16(min(t & 0xFFFFu, t >> 16));
     };
 
-    // Register ring of prefetched cost words, about 64 bytes per lane in
-    // flight so a wave covers many steps of memory latency.  Slot p holds
-    // step base + p; refills are unconditional, clamped to the last step.
-    constexpr int PF = (NW >= 8) ? 2 : 16 / NW;
-    uint32_t buf[PF][NW];
+    // Register ring of prefetched words (the cost, and with FUSE the three
+    // other directions), about 64 bytes per lane in flight so a wave covers
+    // many steps of memory latency.  Slot p holds step base + p; refills are
+    // unconditional, clamped to the last step.
+    constexpr int NSRC = FUSE ? 4 : 1;
+    constexpr int PF = FUSE ? ((NW >= 4) ? 2 : 4 / NW) : ((NW >= 8) ? 2 : 16 / NW);
+    const uint8_t* srcs[4] = {src0, J.path[0] + off0, J.path[1] + off0, J.path[3] + off0};
+    uint32_t buf[PF][NSRC][NW];
 #pragma unroll
-    for (int p = 0; p < PF; ++p) load_words<NW>(src0 + (int64_t)min(1 + p, n - 1) * step, buf[p]);
+    for (int p = 0; p < PF; ++p)
+#pragma unroll
+        for (int q = 0; q < NSRC; ++q)
+            load_words<NW>(srcs[q] + (int64_t)min(1 + p, n - 1) * step, buf[p][q]);
+    auto consume = [&](uint32_t (&cur)[NSRC][NW], int i) __attribute__((always_inline)) {
+        advance(cur[0], i);
+        if constexpr (FUSE) {
+            uint32_t other[3][NW];
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int k = 0; k < NW; ++k) other[q][k] = cur[q + 1][k];
+            wta_step(other, i);
+        }
+    };
     int i = 1;
     for (; i + PF <= n; i += PF) {
 #pragma unroll
         for (int p = 0; p < PF; ++p) {
-            uint32_t c8[NW];
+            uint32_t cur[NSRC][NW];
 #pragma unroll
-            for (int k = 0; k < NW; ++k) c8[k] = buf[p][k];
-            load_words<NW>(src0 + (int64_t)min(i + p + PF, n - 1) * step, buf[p]);
-            advance(c8, i + p);
+            for (int q = 0; q < NSRC; ++q)
+#pragma unroll
+                for (int k = 0; k < NW; ++k) cur[q][k] = buf[p][q][k];
+#pragma unroll
+            for (int q = 0; q < NSRC; ++q)
+                load_words<NW>(srcs[q] + (int64_t)min(i + p + PF, n - 1) * step, buf[p][q]);
+            consume(cur, i + p);
         }
     }
     for (; i < n; ++i) {
-        uint32_t c8[NW];
-        load_words<NW>(src0 + (int64_t)i * step, c8);
-        advance(c8, i);
+        uint32_t cur[NSRC][NW];
+#pragma unroll
+        for (int q = 0; q < NSRC; ++q) load_words<NW>(srcs[q] + (int64_t)i * step, cur[q]);
+        consume(cur, i);
     }
 }
 
-template <int DMAX>
+// MODE 0: all four directions (stage entry point); MODE 1: left->right,
+// right->left, top->bottom; MODE 2: bottom->top fused with the WTA.
+template <int DMAX, int MODE>
 __global__ __launch_bounds__(64) void aggregate_kernel(const SgmJob* __restrict__ jobs,
                                                        const int* __restrict__ offs, int njobs) {
     const int item = blockIdx.x;
@@ -332,17 +422,19 @@ __global__ __launch_bounds__(64) void aggregate_kernel(const SgmJob* __restrict_
     const int local = item - offs[j];
     const int nh = (J.H + 3) / 4, nv = (J.W - J.roi_x + 3) / 4;
     int path, group;
+    if (MODE == 2) { path = kDU; group = local; }
     // long horizontal scanlines are issued first
-    if (local < nh) { path = kLR; group = local; }
+    else if (local < nh) { path = kLR; group = local; }
     else if (local < 2 * nh) { path = kRL; group = local - nh; }
     else if (local < 2 * nh + nv) { path = kUD; group = local - 2 * nh; }
     else { path = kDU; group = local - 2 * nh - nv; }
     const int lane = threadIdx.x;
+    constexpr bool F = (MODE == 2);
     switch (J.D) {
-        case 64: agg_lines<64>(J, path, group, lane); break;
-        case 128: if constexpr (DMAX >= 128) agg_lines<128>(J, path, group, lane); break;
-        case 256: if constexpr (DMAX >= 256) agg_lines<256>(J, path, group, lane); break;
-        case 512: if constexpr (DMAX >= 512) agg_lines<512>(J, path, group, lane); break;
+        case 64: agg_lines<64, F>(J, path, group, lane); break;
+        case 128: if constexpr (DMAX >= 128) agg_lines<128, F>(J, path, group, lane); break;
+        case 256: if constexpr (DMAX >= 256) agg_lines<256, F>(J, path, group, lane); break;
+        case 512: if constexpr (DMAX >= 512) agg_lines<512, F>(J, path, group, lane); break;
         default: break;
     }
 }
@@ -436,13 +528,18 @@ __global__ __launch_bounds__(256) void wta_kernel(const SgmJob* __restrict__ job
 // LR check (cusgm_ca.cu:3873-3898) evaluated per pixel against the right
 // map as it was before the check, then the 3x3 median (cusgm_df.cu:16-75)
 // out of place; x = 0 / y = 0 pass through, outside neighbours are INVALID.
+// columns below roi_x are never matched: INVALID whatever the raw map holds
+__device__ __forceinline__ float raw_at(const float* m, const SgmPost& P, int x, int y) {
+    return (x < P.roi_x) ? kInvalidDisp : m[(int64_t)y * P.W + x];
+}
+
 __device__ __forceinline__ float lr_value(const SgmPost& P, int x, int y) {
-    const float dl = P.raw_l[(int64_t)y * P.W + x];
+    const float dl = raw_at(P.raw_l, P, x, y);
     if (!P.do_lr || dl == kInvalidDisp) return dl;
     const float xrf = (float)x - dl;
     const int xr = (xrf > 0) ? (int)((double)xrf + 0.5) : (int)((double)xrf - 0.5);
     if (xr < 0 || xr >= P.W) return kInvalidDisp;
-    const float dr = P.raw_r[(int64_t)y * P.W + xr];
+    const float dr = raw_at(P.raw_r, P, xr, y);
     if (fabsf(dl - dr) > P.lr_thr) return kInvalidDisp;
     return (dl + dr) / 2.0f;
 }
@@ -485,7 +582,7 @@ __global__ __launch_bounds__(256) void post_kernel(const SgmPost* __restrict__ p
     // the right output starts as the right map (LR invalidations follow in a
     // second launch) or, without the LR check, as all INVALID
     if (P.out_r)
-        P.out_r[(int64_t)y * P.W + x] = P.do_lr ? P.raw_r[(int64_t)y * P.W + x] : kInvalidDisp;
+        P.out_r[(int64_t)y * P.W + x] = P.do_lr ? raw_at(P.raw_r, P, x, y) : kInvalidDisp;
     float v = t[ty + 1][tx + 1];
     if (P.do_median && x > 0 && y > 0)
         v = median9(t[ty][tx], t[ty][tx + 1], t[ty][tx + 2], t[ty + 1][tx], t[ty + 1][tx + 1],
@@ -508,12 +605,12 @@ __global__ __launch_bounds__(256) void right_invalidate_kernel(const SgmPost* __
     const int x = (local % tiles_x) * 16 + (threadIdx.x & 15);
     const int y = (local / tiles_x) * 16 + (threadIdx.x >> 4);
     if (x >= P.W || y >= P.H) return;
-    const float dl = P.raw_l[(int64_t)y * P.W + x];
+    const float dl = raw_at(P.raw_l, P, x, y);
     if (dl == kInvalidDisp) return;
     const float xrf = (float)x - dl;
     const int xr = (xrf > 0) ? (int)((double)xrf + 0.5) : (int)((double)xrf - 0.5);
     if (xr < 0 || xr >= P.W) return;
-    const float dr = P.raw_r[(int64_t)y * P.W + xr];
+    const float dr = raw_at(P.raw_r, P, xr, y);
     if (fabsf(dl - dr) > P.lr_thr) P.out_r[(int64_t)y * P.W + xr] = kInvalidDisp;
 }
 
@@ -544,18 +641,31 @@ hipError_t launch_cost(const StagePlan& p, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_aggregate(const StagePlan& p, hipStream_t s) {
-    if (p.n_agg <= 0) return hipSuccess;
-    const dim3 g(p.n_agg), b(64);
+template <int MODE>
+static hipError_t launch_agg_mode(const StagePlan& p, const int* offs, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const dim3 g(n), b(64);
     if (p.dmax <= 64)
-        hipLaunchKernelGGL(aggregate_kernel<64>, g, b, 0, s, p.jobs, p.offs_agg, p.njobs);
+        hipLaunchKernelGGL((aggregate_kernel<64, MODE>), g, b, 0, s, p.jobs, offs, p.njobs);
     else if (p.dmax <= 128)
-        hipLaunchKernelGGL(aggregate_kernel<128>, g, b, 0, s, p.jobs, p.offs_agg, p.njobs);
+        hipLaunchKernelGGL((aggregate_kernel<128, MODE>), g, b, 0, s, p.jobs, offs, p.njobs);
     else if (p.dmax <= 256)
-        hipLaunchKernelGGL(aggregate_kernel<256>, g, b, 0, s, p.jobs, p.offs_agg, p.njobs);
+        hipLaunchKernelGGL((aggregate_kernel<256, MODE>), g, b, 0, s, p.jobs, offs, p.njobs);
     else
-        hipLaunchKernelGGL(aggregate_kernel<512>, g, b, 0, s, p.jobs, p.offs_agg, p.njobs);
+        hipLaunchKernelGGL((aggregate_kernel<512, MODE>), g, b, 0, s, p.jobs, offs, p.njobs);
     return hipGetLastError();
+}
+
+hipError_t launch_aggregate(const StagePlan& p, hipStream_t s) {
+    return launch_agg_mode<0>(p, p.offs_agg, p.n_agg, s);
+}
+
+hipError_t launch_aggregate3(const StagePlan& p, hipStream_t s) {
+    return launch_agg_mode<1>(p, p.offs_agg3, p.n_agg3, s);
+}
+
+hipError_t launch_aggregate_wta(const StagePlan& p, hipStream_t s) {
+    return launch_agg_mode<2>(p, p.offs_dw, p.n_dw, s);
 }
 
 hipError_t launch_wta(const StagePlan& p, hipStream_t s) {
