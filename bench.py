@@ -307,8 +307,17 @@ def uneven(args):
                       "min_disparity": 0, "disp_range": d})
         cells += w * h * d
     batch = SgmBatch(pairs, opt)
-    if getattr(args, "graph", True) and hasattr(batch, "capture"):
-        batch.capture()
+    # grid-schedule eager time, then the autotuned schedule run as one graph
+    for _ in range(max(1, args.warmup)):
+        batch.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    torch.cuda.synchronize()
+    grid_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    policy, waves, tuned_ms = batch.autotune(reps=3, force=True)
+    batch.use_graph(True)
     for _ in range(args.warmup):
         batch.run()
     torch.cuda.synchronize()
@@ -349,6 +358,8 @@ def uneven(args):
                    "pairs": 64, "path": "batched"},
                   dtype="u8", plain_value=cells / (p_ms * 1e-3), plain_ms=p_ms,
                   batched_over_plain=p_ms / b_ms, batched_bit_exact_vs_plain=same,
+                  schedule={"policy": ["grid", "job-major", "longest-first"][policy], "waves": waves,
+                            "autotune_ms": tuned_ms, "grid_eager_ms": grid_ms, "graph": True},
                   device_bytes=batch.device_bytes))
 
 

@@ -92,6 +92,11 @@ SIGNATURES = {
     "sgm_batch_stage_times": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float),
                                              ctypes.POINTER(I32)]),
     "sgm_batch_device_bytes": (ctypes.c_size_t, [P]),
+    "sgm_batch_set_schedule": (ctypes.c_int, [P, I32, I32]),
+    "sgm_batch_get_schedule": (ctypes.c_int, [P, ctypes.POINTER(I32), ctypes.POINTER(I32)]),
+    "sgm_batch_autotune": (ctypes.c_int, [P, P, I32, I32, ctypes.POINTER(I32), ctypes.POINTER(I32),
+                                          ctypes.POINTER(ctypes.c_float)]),
+    "sgm_batch_use_graph": (ctypes.c_int, [P, I32]),
     "sgm_batch_destroy": (None, [P]),
     "sgm_dsgm_checkpoint_floats": (ctypes.c_size_t, [I32, I32, I32, I32]),
     "sgm_dsgm_workspace_floats": (ctypes.c_size_t, [I32, I32, I32, I32]),

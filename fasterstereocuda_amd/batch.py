@@ -18,6 +18,9 @@ from .stereo import CuSGMOption
 
 STAGES = ("census", "cost", "aggregate3", "aggregate_wta", "post")
 
+# scanline schedules of the two aggregation launches
+SCHED_GRID, SCHED_JOB_MAJOR, SCHED_LONGEST_FIRST = 0, 1, 2
+
 
 class SgmBatch:
     def __init__(self, pairs, option: CuSGMOption):
@@ -56,6 +59,32 @@ class SgmBatch:
             import torch
             stream = torch.cuda.current_stream().cuda_stream
         check(lib().sgm_batch_run(self._b, stream))
+
+    def set_schedule(self, policy: int, waves: int = 0) -> None:
+        check(lib().sgm_batch_set_schedule(self._b, int(policy), int(waves)))
+
+    @property
+    def schedule(self):
+        """(policy, resident waves) of the aggregation launches."""
+        p, w = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(lib().sgm_batch_get_schedule(self._b, ctypes.byref(p), ctypes.byref(w)))
+        return p.value, w.value
+
+    def autotune(self, reps: int = 3, force: bool = False, stream=None):
+        """Times the candidate schedules, keeps the fastest (remembered per
+        batch signature); returns (policy, waves, ms per run; -1 if the
+        remembered choice was applied without timing)."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        p, w, ms = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_float(0)
+        check(lib().sgm_batch_autotune(self._b, stream, int(reps), int(force), ctypes.byref(p),
+                                       ctypes.byref(w), ctypes.byref(ms)))
+        return p.value, w.value, ms.value
+
+    def use_graph(self, on: bool = True) -> None:
+        """Run the whole pipeline as one captured HIP graph."""
+        check(lib().sgm_batch_use_graph(self._b, int(on)))
 
     def set_timing(self, on: bool) -> None:
         check(lib().sgm_batch_set_timing(self._b, int(on)))

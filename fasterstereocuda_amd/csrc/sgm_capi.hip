@@ -10,7 +10,10 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "sgm_capi.h"
@@ -154,7 +157,7 @@ void fsgm::restrict_matching(StagePlan* P, const std::vector<SgmJob>& jobs, int 
 // invalidation]; marks[k] (may be null) are recorded before stage k and
 // after the last one.
 hipError_t fsgm::launch_match(const StagePlan& plan, bool right_invalidation, hipStream_t s,
-                              hipEvent_t* marks) {
+                              hipEvent_t* marks, const AggSchedule* sched) {
     StagePlan ctw = plan;
     if (plan.njobs_ctw > 0) ctw.njobs = plan.njobs_ctw;
     hipError_t e;
@@ -164,9 +167,12 @@ hipError_t fsgm::launch_match(const StagePlan& plan, bool right_invalidation, hi
     if ((e = mark(1)) != hipSuccess) return e;
     if ((e = launch_cost(ctw, s)) != hipSuccess) return e;
     if ((e = mark(2)) != hipSuccess) return e;
-    if ((e = launch_aggregate3(ctw, s)) != hipSuccess) return e;
+    if ((e = sched ? launch_aggregate_sched(ctw, *sched, s) : launch_aggregate3(ctw, s)) != hipSuccess)
+        return e;
     if ((e = mark(3)) != hipSuccess) return e;
-    if ((e = launch_aggregate_wta(ctw, s)) != hipSuccess) return e;
+    if ((e = sched ? launch_aggregate_wta_sched(ctw, *sched, s) : launch_aggregate_wta(ctw, s)) !=
+        hipSuccess)
+        return e;
     if ((e = mark(4)) != hipSuccess) return e;
     if ((e = launch_post(plan, s)) != hipSuccess) return e;
     if (right_invalidation && (e = launch_right_invalidate(plan, s)) != hipSuccess) return e;
@@ -455,6 +461,18 @@ struct sgm_batch {
     std::vector<hipEvent_t> pool;     // free events
     std::vector<hipEvent_t> marks;    // 6 per recorded run
     std::string err;
+    // scanline schedule
+    std::vector<SgmJob> jobs;         // host copy of the matched jobs
+    std::string signature;
+    int policy = SGM_SCHED_GRID;
+    int waves = 0;                    // resident workgroups of the persistent launches
+    int per_cu = 0, n_cu = 0;
+    void* sched_block = nullptr;      // items (agg3 then dw) + 2 counters
+    AggSchedule sched;
+    // graph mode
+    bool use_graph = false;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -470,6 +488,9 @@ hipError_t take_event(sgm_batch* b, hipEvent_t* e) {
 
 void free_batch(sgm_batch* b) {
     if (b->device >= 0) (void)hipSetDevice(b->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    if (b->sched_block) (void)hipFree(b->sched_block);
     for (hipEvent_t e : b->marks) (void)hipEventDestroy(e);
     for (hipEvent_t e : b->pool) (void)hipEventDestroy(e);
     if (b->arena) (void)hipFree(b->arena);
@@ -477,6 +498,124 @@ void free_batch(sgm_batch* b) {
     delete b;
 }
 
+
+// Schedules remembered per batch signature (process-wide).
+std::mutex g_sched_mu;
+std::map<std::string, std::pair<int, int>> g_sched_cache;
+
+std::string batch_signature(int device, const sgm_option& o, const sgm_pair_desc* pairs, int n,
+                            bool want_right) {
+    std::string s = "dev" + std::to_string(device) + "|cs" + std::to_string(o.cs_type) + "|p" +
+                    std::to_string(o.p1) + "," + std::to_string(o.p2_init) + "|lr" +
+                    std::to_string(o.do_lr_check) + "|r" + std::to_string(want_right ? 1 : 0);
+    for (int i = 0; i < n; ++i)
+        s += "|" + std::to_string(pairs[i].width) + "x" + std::to_string(pairs[i].height) + "d" +
+             std::to_string(pairs[i].min_disparity) + "+" + std::to_string(pairs[i].disp_range);
+    return s;
+}
+
+// Packed aggregation items (job << 16 | direction << 14 | group) of the
+// matched jobs, in the order `policy` asks for.  Directions 0..3 are
+// top->bottom, left->right, bottom->top, right->left.
+void build_items(const std::vector<SgmJob>& jobs, int policy, std::vector<uint32_t>* agg3,
+                 std::vector<uint32_t>* dw) {
+    struct Item { int64_t work; uint32_t code; };
+    std::vector<Item> a, b;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        const SgmJob& J = jobs[j];
+        const int nh = (J.H + 3) / 4, wv = J.W - J.roi_x, nv = (wv + 3) / 4;
+        auto code = [&](int dir, int g) { return (uint32_t)j << 16 | (uint32_t)dir << 14 | (uint32_t)g; };
+        for (int dir : {1, 3})   // left->right, right->left
+            for (int g = 0; g < nh; ++g)
+                a.push_back({(int64_t)std::min(4, J.H - 4 * g) * wv * J.D, code(dir, g)});
+        for (int g = 0; g < nv; ++g)
+            a.push_back({(int64_t)std::min(4, wv - 4 * g) * J.H * J.D, code(0, g)});
+        for (int g = 0; g < nv; ++g)
+            b.push_back({(int64_t)std::min(4, wv - 4 * g) * J.H * J.D, code(2, g)});
+    }
+    if (policy == SGM_SCHED_LONGEST_FIRST) {
+        auto by_work = [](const Item& x, const Item& y) { return x.work > y.work; };
+        std::stable_sort(a.begin(), a.end(), by_work);
+        std::stable_sort(b.begin(), b.end(), by_work);
+    }
+    agg3->clear();
+    dw->clear();
+    for (const Item& x : a) agg3->push_back(x.code);
+    for (const Item& x : b) dw->push_back(x.code);
+}
+
+void drop_graph(sgm_batch* b) {
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+}
+
+int apply_schedule(sgm_batch* b, int policy, int waves) {
+    if (policy < SGM_SCHED_GRID || policy > SGM_SCHED_LONGEST_FIRST || waves < 0)
+        return fail(&b->err, SGM_E_INVALID_ARG, "unknown schedule policy or negative wave count");
+    drop_graph(b);
+    if (policy == SGM_SCHED_GRID) {
+        b->policy = policy;
+        b->waves = 0;
+        return SGM_OK;
+    }
+    if (b->jobs.size() > 65535)
+        return fail(&b->err, SGM_E_INVALID_ARG, "persistent schedules take at most 65535 views");
+    for (const SgmJob& J : b->jobs)
+        if (J.H > 65532 || J.W > 65532)
+            return fail(&b->err, SGM_E_INVALID_ARG, "persistent schedules take sides up to 65532");
+    std::vector<uint32_t> a3, dw;
+    build_items(b->jobs, policy, &a3, &dw);
+    const size_t bytes = (a3.size() + dw.size() + 2) * sizeof(uint32_t);
+    HIP_TRY(&b->err, hipSetDevice(b->device));
+    if (!b->sched_block) HIP_TRY(&b->err, hipMalloc(&b->sched_block, bytes));
+    std::vector<uint32_t> host(a3);
+    host.insert(host.end(), dw.begin(), dw.end());
+    host.push_back(0u);
+    host.push_back(0u);
+    // earlier runs may still read the old list
+    HIP_TRY(&b->err, hipDeviceSynchronize());
+    HIP_TRY(&b->err, hipMemcpy(b->sched_block, host.data(), bytes, hipMemcpyHostToDevice));
+    uint32_t* base = static_cast<uint32_t*>(b->sched_block);
+    b->sched.items3 = base;
+    b->sched.n3 = (int)a3.size();
+    b->sched.items_dw = base + a3.size();
+    b->sched.ndw = (int)dw.size();
+    b->sched.counters = base + a3.size() + dw.size();
+    b->policy = policy;
+    b->waves = waves > 0 ? waves : b->per_cu * b->n_cu;
+    b->sched.grid = b->waves;
+    return SGM_OK;
+}
+
+hipError_t enqueue(sgm_batch* b, hipStream_t s, hipEvent_t* marks) {
+    return launch_match(b->plan, b->want_right && b->opt.do_lr_check, s, marks,
+                        b->policy == SGM_SCHED_GRID ? nullptr : &b->sched);
+}
+
+int run_graph(sgm_batch* b, hipStream_t s) {
+    if (!b->exec) {
+        if (!b->cap_stream) HIP_TRY(&b->err, hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+        HIP_TRY(&b->err, hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = enqueue(b, b->cap_stream, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(b->cap_stream, &g);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(&b->err, SGM_E_HIP, std::string("graph capture failed: ") +
+                                                 hipGetErrorString(le != hipSuccess ? le : ce));
+        }
+        const hipError_t ie = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            b->exec = nullptr;
+            return fail(&b->err, SGM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        }
+    }
+    HIP_TRY(&b->err, hipGraphLaunch(b->exec, s));
+    return SGM_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -570,6 +709,24 @@ sgm_batch* sgm_batch_create(const sgm_pair_desc* pairs, int32_t n, const sgm_opt
         return nullptr;
     }
     if (!lr) restrict_matching(&b->plan, jobs, n);   // only the left views are matched
+    // the matched jobs: all of them with the LR check, else the n left views
+    b->jobs.assign(jobs.begin(), jobs.begin() + (lr ? jobs.size() : (size_t)n));
+    b->signature = batch_signature(b->device, *opt, pairs, n, b->want_right);
+    if (hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
+        persistent_occupancy(b->plan, &b->per_cu) != hipSuccess) {
+        fail(nullptr, SGM_E_HIP, "device query failed");
+        free_batch(b);
+        return nullptr;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_sched_mu);
+        auto it = g_sched_cache.find(b->signature);
+        if (it != g_sched_cache.end() && apply_schedule(b, it->second.first, it->second.second)) {
+            g_last_error = b->err;
+            free_batch(b);
+            return nullptr;
+        }
+    }
     return b;
 }
 
@@ -578,19 +735,112 @@ int sgm_batch_run(sgm_batch* b, void* stream) {
     HIP_TRY(&b->err, hipSetDevice(b->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t ev[6] = {};
+    if (b->use_graph) return run_graph(b, s);
     if (b->timing) {
         for (int k = 0; k < 6; ++k) HIP_TRY(&b->err, take_event(b, &ev[k]));
         b->marks.insert(b->marks.end(), ev, ev + 6);
     }
-    HIP_TRY(&b->err, launch_match(b->plan, b->want_right && b->opt.do_lr_check, s,
-                                  b->timing ? ev : nullptr));
+    HIP_TRY(&b->err, enqueue(b, s, b->timing ? ev : nullptr));
     return SGM_OK;
 }
 
 int sgm_batch_set_timing(sgm_batch* b, int32_t on) {
     if (!b) return fail(nullptr, SGM_E_INVALID_ARG, "batch is NULL");
+    if (on && b->use_graph) return fail(&b->err, SGM_E_INVALID_ARG, "timing is off in graph mode");
     b->timing = on != 0;
     return SGM_OK;
+}
+
+int sgm_batch_use_graph(sgm_batch* b, int32_t on) {
+    if (!b) return fail(nullptr, SGM_E_INVALID_ARG, "batch is NULL");
+    if (on && b->timing) return fail(&b->err, SGM_E_INVALID_ARG, "timing is off in graph mode");
+    b->use_graph = on != 0;
+    return SGM_OK;
+}
+
+int sgm_batch_set_schedule(sgm_batch* b, int32_t policy, int32_t waves) {
+    if (!b) return fail(nullptr, SGM_E_INVALID_ARG, "batch is NULL");
+    return apply_schedule(b, policy, waves);
+}
+
+int sgm_batch_get_schedule(const sgm_batch* b, int32_t* policy, int32_t* waves) {
+    if (!b || !policy || !waves) return fail(nullptr, SGM_E_INVALID_ARG, "NULL argument");
+    *policy = b->policy;
+    *waves = b->waves;
+    return SGM_OK;
+}
+
+int sgm_batch_autotune(sgm_batch* b, void* stream, int32_t reps, int32_t force, int32_t* policy,
+                       int32_t* waves, float* ms_per_run) {
+    if (!b) return fail(nullptr, SGM_E_INVALID_ARG, "batch is NULL");
+    if (reps <= 0) return fail(&b->err, SGM_E_INVALID_ARG, "reps must be positive");
+    HIP_TRY(&b->err, hipSetDevice(b->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto report = [&](float ms) {
+        if (policy) *policy = b->policy;
+        if (waves) *waves = b->waves;
+        if (ms_per_run) *ms_per_run = ms;
+        return SGM_OK;
+    };
+    if (!force) {
+        std::pair<int, int> hit{-1, 0};
+        {
+            std::lock_guard<std::mutex> lk(g_sched_mu);
+            auto it = g_sched_cache.find(b->signature);
+            if (it != g_sched_cache.end()) hit = it->second;
+        }
+        if (hit.first >= 0) {
+            const int rc = apply_schedule(b, hit.first, hit.second);
+            return rc ? rc : report(-1.f);
+        }
+    }
+    const int full = b->per_cu * b->n_cu;
+    const std::vector<std::pair<int, int>> cand = {
+        {SGM_SCHED_GRID, 0},
+        {SGM_SCHED_LONGEST_FIRST, full},
+        {SGM_SCHED_LONGEST_FIRST, std::max(1, full / 2)},
+        {SGM_SCHED_LONGEST_FIRST, 2 * full},
+        {SGM_SCHED_JOB_MAJOR, full},
+    };
+    const bool was_timing = b->timing, was_graph = b->use_graph;
+    b->timing = false;
+    b->use_graph = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(&b->err, hipEventCreate(&e0));
+    HIP_TRY(&b->err, hipEventCreate(&e1));
+    float best = 0.f;
+    std::pair<int, int> pick = cand[0];
+    int rc = SGM_OK;
+    for (size_t c = 0; c < cand.size() && rc == SGM_OK; ++c) {
+        if ((rc = apply_schedule(b, cand[c].first, cand[c].second)) != SGM_OK) break;
+        hipError_t e = enqueue(b, s, nullptr);   // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = enqueue(b, s, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            rc = fail(&b->err, SGM_E_HIP, std::string("autotune run: ") + hipGetErrorString(e));
+            break;
+        }
+        ms /= (float)reps;
+        if (c == 0 || ms < best) {
+            best = ms;
+            pick = {cand[c].first, b->waves};
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->timing = was_timing;
+    b->use_graph = was_graph;
+    if (rc != SGM_OK) return rc;
+    if ((rc = apply_schedule(b, pick.first, pick.second)) != SGM_OK) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_sched_mu);
+        g_sched_cache[b->signature] = pick;
+    }
+    return report(best);
 }
 
 int sgm_batch_stage_times(sgm_batch* b, float* ms, int32_t* runs) {

@@ -74,12 +74,27 @@ struct StagePlan {
     int dmax = 64;
 };
 
+// Scanline schedule of the persistent aggregation launches: packed items
+// (job << 16 | direction << 14 | group) in execution order, two counters.
+struct AggSchedule {
+    const uint32_t* items3 = nullptr;   // LR, RL, UD items
+    int n3 = 0;
+    const uint32_t* items_dw = nullptr; // bottom->top + WTA items
+    int ndw = 0;
+    uint32_t* counters = nullptr;       // [2], reset by the first launch
+    int grid = 0;                       // resident waves
+};
+
 // Launchers (sgm_kernels.hip).  All enqueue on `stream`.
 hipError_t launch_census(const StagePlan& p, hipStream_t s);
 hipError_t launch_cost(const StagePlan& p, hipStream_t s);
 hipError_t launch_aggregate(const StagePlan& p, hipStream_t s);        // 4 directions
 hipError_t launch_aggregate3(const StagePlan& p, hipStream_t s);       // LR, RL, UD
 hipError_t launch_aggregate_wta(const StagePlan& p, hipStream_t s);    // DU + WTA
+hipError_t launch_aggregate_sched(const StagePlan& p, const AggSchedule& sc, hipStream_t s);
+hipError_t launch_aggregate_wta_sched(const StagePlan& p, const AggSchedule& sc, hipStream_t s);
+// Resident 64-lane workgroups per compute unit of the persistent kernels.
+hipError_t persistent_occupancy(const StagePlan& p, int* per_cu);
 hipError_t launch_wta(const StagePlan& p, hipStream_t s);
 hipError_t launch_post(const StagePlan& p, hipStream_t s);
 hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s);
@@ -89,6 +104,6 @@ hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t s);
 // pyramid paths (marks: 6 events or null)
 void restrict_matching(StagePlan* P, const std::vector<SgmJob>& jobs, int n);
 hipError_t launch_match(const StagePlan& plan, bool right_invalidation, hipStream_t s,
-                        hipEvent_t* marks);
+                        hipEvent_t* marks, const AggSchedule* sched = nullptr);
 
 }  // namespace fsgm

@@ -439,6 +439,38 @@ __global__ __launch_bounds__(64) void aggregate_kernel(const SgmJob* __restrict_
     }
 }
 
+// Persistent form of MODE 1 / MODE 2: a fixed grid of resident waves pulls
+// packed items (job << 16 | direction << 14 | group) from a device counter
+// until the list is exhausted; every wave leaves through that one test.  The
+// list order is the scanline schedule (e.g. longest scanlines first).
+template <int DMAX, int MODE>
+__global__ __launch_bounds__(64) void aggregate_persistent_kernel(
+    const SgmJob* __restrict__ jobs, const uint32_t* __restrict__ items, int n_items,
+    uint32_t* __restrict__ counter) {
+    const int lane = threadIdx.x;
+    constexpr bool F = (MODE == 2);
+    for (;;) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(counter, 1u);
+        idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(idx, 0));
+        if (idx >= (uint32_t)n_items) break;
+        const uint32_t it = items[idx];
+        const SgmJob& J = jobs[it >> 16];
+        const int path = (int)((it >> 14) & 3u), group = (int)(it & 0x3FFFu);
+        switch (J.D) {
+            case 64: agg_lines<64, F>(J, path, group, lane); break;
+            case 128: if constexpr (DMAX >= 128) agg_lines<128, F>(J, path, group, lane); break;
+            case 256: if constexpr (DMAX >= 256) agg_lines<256, F>(J, path, group, lane); break;
+            case 512: if constexpr (DMAX >= 512) agg_lines<512, F>(J, path, group, lane); break;
+            default: break;
+        }
+    }
+}
+
+__global__ void reset_counters_kernel(uint32_t* c, int n) {
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
 // ------------------------------------------------------------------- wta ---
 // cusgm_ca.cu:3606-3699 (D=64), :3701-3809 (D=128), :3149-3232 (generic),
 // SubPixBias :150-154.  16 lanes per pixel, D/16 disparities per lane.
@@ -666,6 +698,55 @@ hipError_t launch_aggregate3(const StagePlan& p, hipStream_t s) {
 
 hipError_t launch_aggregate_wta(const StagePlan& p, hipStream_t s) {
     return launch_agg_mode<2>(p, p.offs_dw, p.n_dw, s);
+}
+
+template <int MODE>
+static hipError_t launch_persistent_mode(const StagePlan& p, const uint32_t* items, int n,
+                                         uint32_t* counter, int grid, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const dim3 g(grid), b(64);
+    if (p.dmax <= 64)
+        hipLaunchKernelGGL((aggregate_persistent_kernel<64, MODE>), g, b, 0, s, p.jobs, items, n, counter);
+    else if (p.dmax <= 128)
+        hipLaunchKernelGGL((aggregate_persistent_kernel<128, MODE>), g, b, 0, s, p.jobs, items, n, counter);
+    else if (p.dmax <= 256)
+        hipLaunchKernelGGL((aggregate_persistent_kernel<256, MODE>), g, b, 0, s, p.jobs, items, n, counter);
+    else
+        hipLaunchKernelGGL((aggregate_persistent_kernel<512, MODE>), g, b, 0, s, p.jobs, items, n, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_aggregate_sched(const StagePlan& p, const AggSchedule& sc, hipStream_t s) {
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, sc.counters, 2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = launch_persistent_mode<1>(p, sc.items3, sc.n3, sc.counters, sc.grid, s)) != hipSuccess)
+        return e;
+    return hipSuccess;
+}
+
+hipError_t launch_aggregate_wta_sched(const StagePlan& p, const AggSchedule& sc, hipStream_t s) {
+    return launch_persistent_mode<2>(p, sc.items_dw, sc.ndw, sc.counters + 1, sc.grid, s);
+}
+
+template <int MODE>
+static hipError_t occ_mode(int dmax, int* n) {
+    if (dmax <= 64)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, aggregate_persistent_kernel<64, MODE>, 64, 0);
+    if (dmax <= 128)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, aggregate_persistent_kernel<128, MODE>, 64, 0);
+    if (dmax <= 256)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, aggregate_persistent_kernel<256, MODE>, 64, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, aggregate_persistent_kernel<512, MODE>, 64, 0);
+}
+
+hipError_t persistent_occupancy(const StagePlan& p, int* per_cu) {
+    int a = 0, b = 0;
+    hipError_t e = occ_mode<1>(p.dmax, &a);
+    if (e != hipSuccess) return e;
+    if ((e = occ_mode<2>(p.dmax, &b)) != hipSuccess) return e;
+    *per_cu = std::max(1, std::min(a, b));
+    return hipSuccess;
 }
 
 hipError_t launch_wta(const StagePlan& p, hipStream_t s) {

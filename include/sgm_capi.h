@@ -132,10 +132,37 @@ int sgm_batch_run(sgm_batch* b, void* stream);
 /* With timing on, every run records HIP events between its stages. */
 int sgm_batch_set_timing(sgm_batch* b, int32_t on);
 /* Sums of the per-stage times (ms) of the runs recorded since the last call:
- * ms[0..4] = census, cost, aggregate, wta, post.  Waits for those runs. */
+ * ms[0..4] = census, cost, aggregate (left->right, right->left, top->bottom),
+ * aggregate bottom->top fused with the WTA, post.  Waits for those runs. */
 int sgm_batch_stage_times(sgm_batch* b, float* ms, int32_t* runs);
 size_t sgm_batch_device_bytes(const sgm_batch* b);
 void sgm_batch_destroy(sgm_batch* b);
+
+/* Scanline schedule of the two aggregation launches.
+ *   SGM_SCHED_GRID           one workgroup per 4-scanline group (default)
+ *   SGM_SCHED_JOB_MAJOR      persistent: `waves` resident workgroups pull the
+ *                            groups from a device queue, pair by pair
+ *   SGM_SCHED_LONGEST_FIRST  persistent, queue sorted by work (scanline length
+ *                            x disparity range), longest first
+ * waves = 0 picks occupancy x compute units.  Results are identical under
+ * every schedule; only the order in which scanline groups run changes. */
+#define SGM_SCHED_GRID 0
+#define SGM_SCHED_JOB_MAJOR 1
+#define SGM_SCHED_LONGEST_FIRST 2
+int sgm_batch_set_schedule(sgm_batch* b, int32_t policy, int32_t waves);
+int sgm_batch_get_schedule(const sgm_batch* b, int32_t* policy, int32_t* waves);
+/* Times the candidate schedules on `stream` (`reps` runs each, after one
+ * warm-up run), keeps the fastest and remembers it for every later batch of
+ * the same signature (device, options, per-pair size / range) in this
+ * process; a remembered signature is applied without timing unless
+ * `force` != 0.  Writes the chosen policy / waves and its ms per run. */
+int sgm_batch_autotune(sgm_batch* b, void* stream, int32_t reps, int32_t force, int32_t* policy,
+                       int32_t* waves, float* ms_per_run);
+/* With graphs on, sgm_batch_run launches one HIP graph holding the whole
+ * pipeline (captured on a private stream at the first run after a change of
+ * schedule) instead of the individual kernels.  Timing and graphs exclude
+ * each other. */
+int sgm_batch_use_graph(sgm_batch* b, int32_t on);
 
 /* ---- differentiable FP32 SGM (no reference counterpart) ------------------
  * Float cost volume C[B][H][W][D], per-pixel penalties P1, P2 [B][H][W],

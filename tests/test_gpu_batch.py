@@ -76,3 +76,110 @@ def test_batch_rejects_bad_input(cuda):
     with pytest.raises(ValueError):
         SgmBatch([{"left": t.float(), "right": t, "disp_left": o, "disp_range": 64}],
                  CuSGMOption.benchmark())
+
+
+def _uneven_pairs(cuda, lr):
+    scenes_ = [scenes.cone_like(i, H=h, W=w, D=d, workload=3) for i, (w, h, d, _) in enumerate(SPECS)]
+    pairs = []
+    for sc, (w, h, d, m) in zip(scenes_, SPECS):
+        pairs.append({"left": torch.from_numpy(sc.left).to(cuda),
+                      "right": torch.from_numpy(sc.right).to(cuda),
+                      "disp_left": torch.full((h, w), 7.0, device=cuda),
+                      "disp_right": torch.full((h, w), 7.0, device=cuda) if lr else None,
+                      "min_disparity": m, "disp_range": d})
+    return pairs
+
+
+def _outputs(pairs):
+    torch.cuda.synchronize()
+    out = []
+    for p in pairs:
+        out.append(_bits(p["disp_left"].cpu().numpy()))
+        if p["disp_right"] is not None:
+            out.append(_bits(p["disp_right"].cpu().numpy()))
+    return out
+
+
+def _clobber(pairs):
+    for p in pairs:
+        p["disp_left"].fill_(7.0)
+        if p["disp_right"] is not None:
+            p["disp_right"].fill_(7.0)
+
+
+@pytest.mark.parametrize("lr", [True, False])
+def test_schedules_give_identical_bytes(cuda, lr):
+    """Grid, job-major and longest-first schedules, with full and tiny
+    persistent grids (each wave then drains many queue entries), produce the
+    bytes of the grid schedule, run after run (the queue counters restart)."""
+    from fasterstereocuda_amd.batch import SCHED_GRID, SCHED_JOB_MAJOR, SCHED_LONGEST_FIRST
+    pairs = _uneven_pairs(cuda, lr)
+    b = SgmBatch(pairs, CuSGMOption.benchmark(do_lr_check=lr))
+    b.set_schedule(SCHED_GRID)
+    b.run()
+    ref = _outputs(pairs)
+    for policy in (SCHED_JOB_MAJOR, SCHED_LONGEST_FIRST):
+        for waves in (0, 7, 3000):
+            print(f"[schedule] lr={lr} policy={policy} waves={waves}", flush=True)
+            b.set_schedule(policy, waves)
+            got_policy, got_waves = b.schedule
+            assert got_policy == policy and got_waves > 0
+            for _ in range(2):
+                _clobber(pairs)
+                b.run()
+                for k, (g, r) in enumerate(zip(_outputs(pairs), ref)):
+                    np.testing.assert_array_equal(g, r, err_msg=f"policy {policy} waves {waves} map {k}")
+    with pytest.raises(RuntimeError):
+        b.set_schedule(9)
+
+
+@pytest.mark.parametrize("policy", [0, 2])
+def test_graph_replays_match_eager(cuda, policy):
+    """The captured pipeline (kernel nodes only; the queue counters are reset
+    by a kernel) replayed several times, with eager torch work and
+    synchronisation between replays, gives the eager bytes every time."""
+    pairs = _uneven_pairs(cuda, True)
+    b = SgmBatch(pairs, CuSGMOption.benchmark())
+    b.set_schedule(policy)
+    b.run()
+    ref = _outputs(pairs)
+    b.use_graph(True)
+    with pytest.raises(RuntimeError, match="graph mode"):
+        b.set_timing(True)
+    scratch = torch.randn(1 << 20, device=cuda)
+    for rep in range(4):
+        print(f"[graph] policy={policy} replay {rep}", flush=True)
+        _clobber(pairs)
+        scratch.mul_(0.5).add_(1.0)           # eager work between replays
+        torch.cuda.synchronize()
+        b.run()
+        for k, (g, r) in enumerate(zip(_outputs(pairs), ref)):
+            np.testing.assert_array_equal(g, r, err_msg=f"replay {rep} map {k}")
+    b.use_graph(False)
+    _clobber(pairs)
+    b.run()
+    for g, r in zip(_outputs(pairs), ref):
+        np.testing.assert_array_equal(g, r)
+
+
+def test_autotune_picks_and_remembers(cuda):
+    pairs = _uneven_pairs(cuda, True)
+    b = SgmBatch(pairs, CuSGMOption.benchmark())
+    b.run()
+    ref = _outputs(pairs)
+    policy, waves, ms = b.autotune(reps=2, force=True)
+    print(f"[autotune] policy={policy} waves={waves} ms={ms:.3f}", flush=True)
+    assert policy in (0, 1, 2) and ms > 0
+    assert b.schedule == (policy, waves)
+    _clobber(pairs)
+    b.run()
+    for g, r in zip(_outputs(pairs), ref):
+        np.testing.assert_array_equal(g, r)
+    # a second batch of the same signature starts on the remembered schedule
+    pairs2 = _uneven_pairs(cuda, True)
+    b2 = SgmBatch(pairs2, CuSGMOption.benchmark())
+    assert b2.schedule == (policy, waves)
+    assert b2.autotune(reps=2)[2] == -1.0     # cached: applied without timing
+    b2.run()
+    for g, r in zip(_outputs(pairs2), ref):
+        np.testing.assert_array_equal(g, r)
