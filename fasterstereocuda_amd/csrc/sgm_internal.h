@@ -50,7 +50,8 @@ struct SgmPost {                   // one stereo pair: LR check + median
 
 // Items per job for each stage.
 inline int census_items(const SgmJob& j) { return ((j.W + 63) / 64) * ((j.H + 15) / 16); }
-inline int cost_items(const SgmJob& j) { return j.H * ((j.W - j.roi_x + 63) / 64); }
+constexpr int kCostSeg = 256;   // pixels of one row per cost workgroup
+inline int cost_items(const SgmJob& j) { return j.H * ((j.W - j.roi_x + kCostSeg - 1) / kCostSeg); }
 inline int agg_items(const SgmJob& j) { return 2 * ((j.H + 3) / 4) + 2 * ((j.W - j.roi_x + 3) / 4); }
 inline int agg3_items(const SgmJob& j) { return 2 * ((j.H + 3) / 4) + ((j.W - j.roi_x + 3) / 4); }
 inline int dw_items(const SgmJob& j) { return (j.W - j.roi_x + 3) / 4; }

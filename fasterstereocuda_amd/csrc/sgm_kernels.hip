@@ -76,6 +76,18 @@ typedef __attribute__((address_space(1))) uint32_t g_u32;
 typedef __attribute__((address_space(1))) u32x2 g_u2;
 typedef __attribute__((address_space(1))) u32x4 g_u4;
 
+// Scalar element access through the global address space.  Generic (flat)
+// accesses count on both the vector-memory and the LDS counters, which makes
+// the compiler drain every outstanding load before the next use of any.
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+
 template <int NW>
 __device__ __forceinline__ void load_words(const uint8_t* p, uint32_t (&w)[NW]) {
     if constexpr (NW == 1) {
@@ -134,7 +146,7 @@ __global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ 
         const int r = idx / 72, c = idx % 72;
         const int yy = min(max(y0 - 3 + r, 0), J.H - 1);
         const int xx = min(max(x0 - 4 + c, 0), J.W - 1);
-        tile[r][c] = J.img[(int64_t)yy * J.W + xx];
+        tile[r][c] = gld(J.img + (int64_t)yy * J.W + xx);
     }
     __syncthreads();
     const int lx = tid & 63, ly = tid >> 6;
@@ -151,7 +163,7 @@ __global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ 
             v = census_px<4, 3>(tile, ly * 4 + rr + 3, lx + 4);
         else
             v = census_px<2, 2>(tile, ly * 4 + rr + 3, lx + 4);
-        J.census[(int64_t)y * J.W + x] = v;
+        gst(J.census + (int64_t)y * J.W + x, v);
     }
 }
 
@@ -165,56 +177,58 @@ __global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ 
 // of that pixel, which is what is written here.
 __global__ __launch_bounds__(256) void cost_kernel(const SgmJob* __restrict__ jobs,
                                                    const int* __restrict__ offs, int njobs) {
-    __shared__ uint64_t sb[64];
-    __shared__ uint64_t sw[64 + kMaxD];
+    __shared__ uint64_t sb[kCostSeg];
+    __shared__ uint64_t sw[kCostSeg + kMaxD];
     const int item = blockIdx.x;
     const int j = find_job(offs, njobs, item);
     const SgmJob& J = jobs[j];
     const int local = item - offs[j];
-    const int W = J.W, D = J.D, min_d = J.min_d;
-    const int segs = (W - J.roi_x + 63) / 64;
+    const int W = J.W, D = J.D, min_d = J.min_d, roi_x = J.roi_x;
+    const int segs = (W - roi_x + kCostSeg - 1) / kCostSeg;
     const int y = local / segs;
-    const int x0 = J.roi_x + (local % segs) * 64;
+    const int x0 = roi_x + (local - y * segs) * kCostSeg;
     const uint64_t* cb = J.census + (int64_t)y * W;
     const uint64_t* cm = J.census_other + (int64_t)y * W;
     const bool l2r = !J.is_right;
+    const bool quirk_job = J.is_right && J.cs_type == 0;
     const int tid = threadIdx.x;
-    if (tid < 64) sb[tid] = (x0 + tid < W) ? cb[x0 + tid] : 0ull;
+    sb[tid] = (x0 + tid < W) ? gld(cb + x0 + tid) : 0ull;
     const int wbase = l2r ? x0 - min_d - (D - 1) : x0 + min_d;
-    const uint64_t cm0 = cm[0];
-    for (int k = tid; k < 64 + D - 1; k += 256) {
+    const uint64_t cm0 = gld(cm);
+    for (int k = tid; k < kCostSeg + D - 1; k += 256) {
         const int xm = wbase + k;
-        sw[k] = (xm >= 0 && xm < W) ? cm[xm] : cm0;
+        sw[k] = (xm >= 0 && xm < W) ? gld(cm + xm) : cm0;
     }
     __syncthreads();
-    const int groups = D / 16;
-    for (int w = tid; w < 64 * groups; w += 256) {
-        const int i = w / groups, g = w - (w / groups) * groups;
+    // D is a power of two (64..512): groups of 16 disparities per work item
+    const int lg = 31 - __builtin_clz((unsigned)D >> 4);
+    const int gmask = (D >> 4) - 1;
+    const int nx = min(kCostSeg, W - x0);
+    uint8_t* const row_out = J.cost + ((int64_t)y * W + x0) * D;
+    for (int w = tid; w < (nx << lg); w += 256) {
+        const int i = w >> lg, g = w & gmask;
         const int x = x0 + i;
-        if (x >= W) continue;
-        const bool quirk = J.is_right && J.cs_type == 0 &&
-                           (((x - J.roi_x) % D) == D - 1 || x == W - 1);
+        const bool quirk = quirk_job && ((((x - roi_x) & (D - 1)) == D - 1) || x == W - 1);
         uint32_t out[4] = {0u, 0u, 0u, 0u};
         if (!quirk) {
             const uint64_t b = sb[i];
+            const int base = l2r ? i - g * 16 + D - 1 : i + g * 16;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const int t = g * 16 + q;
-                const uint64_t m = l2r ? sw[i - t + D - 1] : sw[i + t];
+                const uint64_t m = l2r ? sw[base - q] : sw[base + q];
                 out[q >> 2] |= (uint32_t)__popcll(b ^ m) << (8 * (q & 3));
             }
         } else {
-            const uint64_t b = cm[x];
-            const uint64_t cb0 = cb[0];
+            const uint64_t b = gld(cm + x);
+            const uint64_t cb0 = gld(cb);
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const int xr = x - min_d - (g * 16 + q);
-                const uint64_t m = (xr >= 0 && xr < W) ? cb[xr] : cb0;
+                const uint64_t m = (xr >= 0 && xr < W) ? gld(cb + xr) : cb0;
                 out[q >> 2] |= (uint32_t)__popcll(b ^ m) << (8 * (q & 3));
             }
         }
-        *reinterpret_cast<uint4*>(J.cost + ((int64_t)y * W + x) * D + g * 16) =
-            make_uint4(out[0], out[1], out[2], out[3]);
+        *(g_u4*)(row_out + (int64_t)w * 16) = u32x4{out[0], out[1], out[2], out[3]};
     }
 }
 
@@ -253,37 +267,75 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
         x0 = roi_x + line; y0 = fwd ? 0 : H - 1;
         step = fwd ? (int64_t)W * D : -(int64_t)W * D;
     }
-    const int64_t off0 = (y0 * W + x0) * D + l * DPD;
-    const uint8_t* const src0 = J.cost + off0;
+    // Addresses = wave-uniform base (first line of the group, scalar
+    // registers) + a small per-lane offset, so the per-step part of every
+    // address is scalar arithmetic.
+    const int line0 = group * 4;
+    const int64_t offL = horiz ? ((int64_t)line0 * W + x0) * D
+                               : (y0 * W + roi_x + line0) * D;
+    const uint32_t voff = (uint32_t)((line - line0) * (horiz ? W * D : D) + l * DPD);
+    const uint8_t* const src0 = J.cost + offL;
     uint8_t* const out_base = (path == 0) ? J.path[0] : (path == 1) ? J.path[1]
                             : (path == 2) ? J.path[2] : J.path[3];
-    uint8_t* const dst0 = out_base + off0;
+    uint8_t* const dst0 = out_base + offL;
     const uint32_t P1v = (uint32_t)J.p1 * 0x10001u;
     const uint32_t P2v = (uint32_t)J.p2 * 0x10001u;
+    const float uthr = 1.0f - J.uniqueness;
+    const int min_d = J.min_d;
 
     uint32_t w[NW];
-    load_words<NW>(src0, w);
+    load_words<NW>(src0 + voff, w);
     uint32_t L[NP];
 #pragma unroll
     for (int k = 0; k < NW; ++k) { L[2 * k] = bytes01(w[k]); L[2 * k + 1] = bytes23(w[k]); }
-    if (active && !FUSE) store_words<NW>(dst0, w);
-    if (FUSE && active && l == 0) J.disp[y0 * W + x0] = kInvalidDisp;   // row H-1: border
+    if (active && !FUSE) store_words<NW>(dst0 + (int64_t)0 + voff, w);
+    if (FUSE && active && l == 0) gst(J.disp + y0 * W + x0, kInvalidDisp);   // row H-1: border
     uint32_t t0 = L[0];
 #pragma unroll
     for (int k = 1; k < NP; ++k) t0 = pk_min(t0, L[k]);
     uint32_t m = row_min16(min(t0 & 0xFFFFu, t0 >> 16));
 
     // WTA of the pixel at step i from this direction's L and the other three
-    // stored directions (FUSE only)
+    // stored directions (FUSE only).  The integer part (sum, argmin with the
+    // smallest d, c1 / c3 gathered with one row OR-reduction) runs every step
+    // and leaves (c2 << 16 | d*, c1 << 16 | c3), equal on every lane of the
+    // row; lane (i - 1) mod 16 keeps it.  Every 16 steps (and at the end of
+    // the line) each lane finishes its own pixel with the float rules, so the
+    // float work is done once per pixel instead of once per lane.
+    uint32_t kb = 0u, kboth = 0u;
+    auto wta_flush = [&](int i0, int cnt) __attribute__((always_inline)) {
+        const int y = (int)y0 - (i0 + l), x = (int)x0;
+        const int ds = (int)(kb & 0xFFFFu);
+        const uint32_t c2 = kb >> 16;
+        constexpr bool warp_rule = (D == 64 || D == 128);
+        const int rows_cov = warp_rule ? 2 * (H / 2) : H;
+        float outv = kInvalidDisp;
+        if (y < rows_cov && !(x > W - 5 || y > H - 5 || x < 5 || y < 5) && ds > 0 &&
+            (warp_rule || ds < D - 1)) {
+            const float c1 = (float)(kboth >> 16);
+            const float c2f = (float)c2;
+            const float c3 = (ds == D - 1) ? (float)(D - 1) : (float)(kboth & 0xFFFFu);
+            const float a = c1 - c2f, b = c3 - c2f;
+            const float uniq = (a < b) ? a : b;
+            if (!(uniq <= c2f * uthr)) {
+                const float t = c1 + c3 - 2.0f * c2f;
+                const float den = (1.0f < t) ? t : 1.0f;
+                const float bias = (float)((double)(c1 - c3 + den) / (double)(2.0f * den));
+                outv = (float)(ds + min_d) + bias;
+            }
+        }
+        if (active && l < cnt) gst(J.disp + (int64_t)y * W + x, outv);
+    };
     auto wta_step = [&](const uint32_t (&other)[3][NW], int i) __attribute__((always_inline)) {
+        // sums of at most 4 x 255 stay inside each u16 half: plain u32 adds
         uint32_t S[NP];
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
             const int wi = k >> 1;
-            uint32_t a = (k & 1) ? bytes23(other[0][wi]) : bytes01(other[0][wi]);
-            a = pk_add(a, (k & 1) ? bytes23(other[1][wi]) : bytes01(other[1][wi]));
-            a = pk_add(a, (k & 1) ? bytes23(other[2][wi]) : bytes01(other[2][wi]));
-            S[k] = pk_add(a, L[k] & 0x00FF00FFu);
+            const uint32_t a = (k & 1) ? bytes23(other[0][wi]) : bytes01(other[0][wi]);
+            const uint32_t b = (k & 1) ? bytes23(other[1][wi]) : bytes01(other[1][wi]);
+            const uint32_t c = (k & 1) ? bytes23(other[2][wi]) : bytes01(other[2][wi]);
+            S[k] = a + b + c + (L[k] & 0x00FF00FFu);
         }
         uint32_t best = 0xFFFFFFFFu;
 #pragma unroll
@@ -294,41 +346,23 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
         }
         best = row_min16(best);
         const int ds = (int)(best & 0xFFFFu);
-        const uint32_t c2 = best >> 16;
-        // gather S(ds-1) and S(ds+1): one owner lane each, OR-reduced over the row
-        const int k1 = ds - 1 - l * DPD, k3 = ds + 1 - l * DPD;
-        uint32_t v1 = 0u, v3 = 0u;
+        // S(ds - 1) and S(ds + 1): one owner lane each, OR-reduced over the row
+        const uint32_t k1 = (uint32_t)(ds - 1 - l * DPD), k3 = k1 + 2u;
+        uint32_t w1 = S[0], w3 = S[0];
 #pragma unroll
-        for (int k = 0; k < NP; ++k) {
-            v1 = ((k1 >> 1) == k) ? S[k] : v1;
-            v3 = ((k3 >> 1) == k) ? S[k] : v3;
+        for (int k = 1; k < NP; ++k) {
+            w1 = ((k1 >> 1) == (uint32_t)k) ? S[k] : w1;
+            w3 = ((k3 >> 1) == (uint32_t)k) ? S[k] : w3;
         }
-        v1 = (k1 & 1) ? (v1 >> 16) : (v1 & 0xFFFFu);
-        v3 = (k3 & 1) ? (v3 >> 16) : (v3 & 0xFFFFu);
-        uint32_t both = ((k1 >= 0 && k1 < DPD) ? (v1 << 16) : 0u) | ((k3 >= 0 && k3 < DPD) ? v3 : 0u);
+        // bit-field offset (k << 4) & 31 selects the u16 half
+        const uint32_t v1 = (k1 < (uint32_t)DPD) ? __builtin_amdgcn_ubfe(w1, k1 << 4, 16) : 0u;
+        const uint32_t v3 = (k3 < (uint32_t)DPD) ? __builtin_amdgcn_ubfe(w3, k3 << 4, 16) : 0u;
+        uint32_t both = (v1 << 16) | v3;
         both |= dpp<0x128>(0u, both);
         both |= dpp<0x124>(0u, both);
         both |= dpp<0x122>(0u, both);
         both |= dpp<0x121>(0u, both);
-        const int y = (int)y0 - i, x = (int)x0;
-        constexpr bool warp_rule = (D == 64 || D == 128);
-        const int rows_cov = warp_rule ? 2 * (H / 2) : H;
-        float outv = kInvalidDisp;
-        if (y < rows_cov && !(x > W - 5 || y > H - 5 || x < 5 || y < 5) && ds > 0 &&
-            (warp_rule || ds < D - 1)) {
-            const float c1 = (float)(both >> 16);
-            const float c2f = (float)c2;
-            const float c3 = (ds == D - 1) ? (float)(D - 1) : (float)(both & 0xFFFFu);
-            const float a = c1 - c2f, b = c3 - c2f;
-            const float uniq = (a < b) ? a : b;
-            if (!(uniq <= c2f * (1.0f - J.uniqueness))) {
-                const float t = c1 + c3 - 2.0f * c2f;
-                const float den = (1.0f < t) ? t : 1.0f;
-                const float bias = (float)((double)(c1 - c3 + den) / (double)(2.0f * den));
-                outv = (float)(ds + J.min_d) + bias;
-            }
-        }
-        if (active && l == 0) J.disp[(int64_t)y * W + x] = outv;
+        if (l == ((i - 1) & 15)) { kb = best; kboth = both; }
     };
 
     // one recurrence step: consumes the cost words of step i, updates L, m
@@ -359,7 +393,7 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
             uint32_t o[NW];
 #pragma unroll
             for (int k = 0; k < NW; ++k) o[k] = pack4(L[2 * k], L[2 * k + 1]);
-            if (active) store_words<NW>(dst0 + (int64_t)i * step, o);
+            if (active) store_words<NW>(dst0 + (int64_t)i * step + voff, o);
         }
         m = row_min16(min(t & 0xFFFFu, t >> 16));
     };
@@ -370,13 +404,18 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
     // unconditional, clamped to the last step.
     constexpr int NSRC = FUSE ? 4 : 1;
     constexpr int PF = FUSE ? ((NW >= 4) ? 2 : 4 / NW) : ((NW >= 8) ? 2 : 16 / NW);
-    const uint8_t* srcs[4] = {src0, J.path[0] + off0, J.path[1] + off0, J.path[3] + off0};
+    const uint8_t* srcs[4] = {src0, J.path[0] + offL, J.path[1] + offL, J.path[3] + offL};
     uint32_t buf[PF][NSRC][NW];
+    // The prologue issues the slots in the loop's own order (the scheduler
+    // may not interleave them): the wait counts at the loop head are the
+    // merge of both paths, and any other order collapses them to zero.
 #pragma unroll
-    for (int p = 0; p < PF; ++p)
+    for (int p = 0; p < PF; ++p) {
 #pragma unroll
         for (int q = 0; q < NSRC; ++q)
-            load_words<NW>(srcs[q] + (int64_t)min(1 + p, n - 1) * step, buf[p][q]);
+            load_words<NW>(srcs[q] + (int64_t)min(1 + p, n - 1) * step + voff, buf[p][q]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     auto consume = [&](uint32_t (&cur)[NSRC][NW], int i) __attribute__((always_inline)) {
         advance(cur[0], i);
         if constexpr (FUSE) {
@@ -388,8 +427,7 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
             wta_step(other, i);
         }
     };
-    int i = 1;
-    for (; i + PF <= n; i += PF) {
+    auto ring_steps = [&](int i) __attribute__((always_inline)) {   // steps i .. i + PF - 1
 #pragma unroll
         for (int p = 0; p < PF; ++p) {
             uint32_t cur[NSRC][NW];
@@ -397,17 +435,34 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
             for (int q = 0; q < NSRC; ++q)
 #pragma unroll
                 for (int k = 0; k < NW; ++k) cur[q][k] = buf[p][q][k];
+            const int64_t so = (int64_t)min(i + p + PF, n - 1) * step;
 #pragma unroll
-            for (int q = 0; q < NSRC; ++q)
-                load_words<NW>(srcs[q] + (int64_t)min(i + p + PF, n - 1) * step, buf[p][q]);
+            for (int q = 0; q < NSRC; ++q) load_words<NW>(srcs[q] + so + voff, buf[p][q]);
             consume(cur, i + p);
         }
+    };
+    int i = 1;
+    if constexpr (FUSE) {
+        // blocks of 16 steps (steps 16b + 1 .. 16b + 16), then their pixels
+        constexpr int kUnroll = (D <= 128) ? 16 / PF : 1;
+        for (; i + 16 <= n; i += 16) {
+#pragma unroll kUnroll
+            for (int s0 = 0; s0 < 16; s0 += PF) ring_steps(i + s0);
+            wta_flush(i, 16);
+        }
+    } else {
+        for (; i + PF <= n; i += PF) ring_steps(i);
     }
     for (; i < n; ++i) {
         uint32_t cur[NSRC][NW];
 #pragma unroll
-        for (int q = 0; q < NSRC; ++q) load_words<NW>(srcs[q] + (int64_t)i * step, cur[q]);
+        for (int q = 0; q < NSRC; ++q) load_words<NW>(srcs[q] + (int64_t)i * step + voff, cur[q]);
         consume(cur, i);
+    }
+    if constexpr (FUSE) {
+        // pixels of the last, partial block of 16 steps
+        const int last = (n - 2) & 15;   // slot of step n - 1
+        if (n > 1 && last != 15) wta_flush(n - 1 - last, last + 1);
     }
 }
 
@@ -531,7 +586,7 @@ __device__ __forceinline__ void wta_pixel(const SgmJob& J, int x, int y, int l,
                 }
             }
         }
-        J.disp[(int64_t)y * W + x] = outv;
+        gst(J.disp + (int64_t)y * W + x, outv);
     }
 }
 
@@ -562,7 +617,7 @@ __global__ __launch_bounds__(256) void wta_kernel(const SgmJob* __restrict__ job
 // out of place; x = 0 / y = 0 pass through, outside neighbours are INVALID.
 // columns below roi_x are never matched: INVALID whatever the raw map holds
 __device__ __forceinline__ float raw_at(const float* m, const SgmPost& P, int x, int y) {
-    return (x < P.roi_x) ? kInvalidDisp : m[(int64_t)y * P.W + x];
+    return (x < P.roi_x) ? kInvalidDisp : gld(m + (int64_t)y * P.W + x);
 }
 
 __device__ __forceinline__ float lr_value(const SgmPost& P, int x, int y) {
@@ -614,12 +669,12 @@ __global__ __launch_bounds__(256) void post_kernel(const SgmPost* __restrict__ p
     // the right output starts as the right map (LR invalidations follow in a
     // second launch) or, without the LR check, as all INVALID
     if (P.out_r)
-        P.out_r[(int64_t)y * P.W + x] = P.do_lr ? raw_at(P.raw_r, P, x, y) : kInvalidDisp;
+        gst(P.out_r + (int64_t)y * P.W + x, P.do_lr ? raw_at(P.raw_r, P, x, y) : kInvalidDisp);
     float v = t[ty + 1][tx + 1];
     if (P.do_median && x > 0 && y > 0)
         v = median9(t[ty][tx], t[ty][tx + 1], t[ty][tx + 2], t[ty + 1][tx], t[ty + 1][tx + 1],
                     t[ty + 1][tx + 2], t[ty + 2][tx], t[ty + 2][tx + 1], t[ty + 2][tx + 2]);
-    P.out_l[(int64_t)y * P.W + x] = v;
+    gst(P.out_l + (int64_t)y * P.W + x, v);
 }
 
 // Right-map invalidation of the LR check (cusgm_ca.cu:3894-3895): every left
@@ -643,7 +698,7 @@ __global__ __launch_bounds__(256) void right_invalidate_kernel(const SgmPost* __
     const int xr = (xrf > 0) ? (int)((double)xrf + 0.5) : (int)((double)xrf - 0.5);
     if (xr < 0 || xr >= P.W) return;
     const float dr = raw_at(P.raw_r, P, xr, y);
-    if (fabsf(dl - dr) > P.lr_thr) P.out_r[(int64_t)y * P.W + xr] = kInvalidDisp;
+    if (fabsf(dl - dr) > P.lr_thr) gst(P.out_r + (int64_t)y * P.W + xr, kInvalidDisp);
 }
 
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ dst, size_t n, float v) {
