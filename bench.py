@@ -186,7 +186,12 @@ def main():
             "kind": "port",
             "sample": f"{done} pairs of the same workload, sequential, {tc:.1f} s",
             "gpu_output_bit_exact_vs_oracle": exact,
+            "cpu_model": cpu_model(),
         }
+
+    # host-pointer path (the reference's calling pattern: copies over PCIe
+    # inside every Match), one pair at a time, both reference image sizes
+    host = None if args.no_plain else host_match_rates(opt)
 
     line = {
         "metric": "W*H*D/s bit-exact SGM (plain vs batched)",
@@ -221,10 +226,45 @@ def main():
         },
         "cpu_baseline": cpu,
         "plain_value": plain,
+        "host_path": host,
         "plain_bit_exact_vs_batched": same_as_plain,
         "stage_ms": stage_avg,
     }
     print(json.dumps(line), flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_match_rates(opt, reps: int = 20):
+    """ms per StereoCuda.Match on host numpy buffers (PCIe copies included)."""
+    import numpy as np
+    from fasterstereocuda_amd import scenes
+    from fasterstereocuda_amd.stereo import StereoCuda
+    out = {}
+    for name, pair in (("cone_450x375_d64", scenes.cone_like(0)),
+                       ("driving_1242x375_d64", scenes.driving(0))):
+        H, W = pair.left.shape
+        st = StereoCuda()
+        assert st.Init(W, H, 0, 64, opt), st.last_error
+        dl = np.empty((H, W), np.float32)
+        dr = np.empty((H, W), np.float32)
+        st.Match(pair.left, pair.right, dl, dr)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            st.Match(pair.left, pair.right, dl, dr)
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        st.Release()
+        out[name] = {"ms_per_pair": ms, "W*H*D/s": W * H * 64 / (ms * 1e-3)}
+    return out
 
 
 def hbm_copy_rate(dev, gib: int = 2, reps: int = 5) -> float:

@@ -1,8 +1,9 @@
 """Builds the native libraries in-tree (no JIT cache, nothing installed).
 
-  fasterstereocuda_amd/libfsgm.so        HIP kernels + C ABI (hipcc, gfx950)
-  fasterstereocuda_amd/libStereoCuda.so  C++ drop-in classes over the C ABI
-  fasterstereocuda_amd/stereo_demo       C++ program that drives StereoCuda
+  fasterstereocuda_amd/libfsgm.so              HIP kernels + C ABI (hipcc, gfx950)
+  fasterstereocuda_amd/libStereoCuda.so        StereoCuda drop-in class over the C ABI
+  fasterstereocuda_amd/libFasterStereoCuda.so  FasterStereoCuda drop-in class over the C ABI
+  fasterstereocuda_amd/stereo_demo             C++ program that drives both classes
 
 Run as ``python -m fasterstereocuda_amd.build``.
 """
@@ -33,6 +34,7 @@ HIPCC_FLAGS = [
 
 LIB = os.path.join(PKG, "libfsgm.so")
 CXX_LIB = os.path.join(PKG, "libStereoCuda.so")
+FS_LIB = os.path.join(PKG, "libFasterStereoCuda.so")
 DEMO = os.path.join(PKG, "stereo_demo")
 
 
@@ -67,15 +69,14 @@ def build_hip(verbose: bool = False) -> str:
 def build_cxx(verbose: bool = False) -> None:
     cxx = shutil.which("g++") or "g++"
     rpath = "-Wl,-rpath,$ORIGIN"
-    tmp = CXX_LIB + ".tmp"
-    srcs = [os.path.join(CSRC, f) for f in ("StereoCuda.cpp", "FasterStereoCuda.cpp")
-            if os.path.exists(os.path.join(CSRC, f))]
-    _run([cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-I", INCLUDE, *srcs,
-          "-L", PKG, "-lfsgm", rpath, "-o", tmp], verbose)
-    os.replace(tmp, CXX_LIB)
+    for out, src in ((CXX_LIB, "StereoCuda.cpp"), (FS_LIB, "FasterStereoCuda.cpp")):
+        tmp = out + ".tmp"
+        _run([cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-I", INCLUDE, os.path.join(CSRC, src),
+              "-L", PKG, "-lfsgm", rpath, "-o", tmp], verbose)
+        os.replace(tmp, out)
     tmp = DEMO + ".tmp"
     _run([cxx, "-O2", "-std=c++17", "-I", INCLUDE, os.path.join(CSRC, "stereo_demo.cpp"),
-          "-L", PKG, "-lStereoCuda", "-lfsgm", rpath, "-o", tmp], verbose)
+          "-L", PKG, "-lStereoCuda", "-lFasterStereoCuda", "-lfsgm", rpath, "-o", tmp], verbose)
     os.replace(tmp, DEMO)
 
 
@@ -86,4 +87,4 @@ def build_all(verbose: bool = False) -> None:
 
 if __name__ == "__main__":
     build_all(verbose="-v" in sys.argv)
-    print("built", LIB, CXX_LIB, DEMO)
+    print("built", LIB, CXX_LIB, FS_LIB, DEMO)

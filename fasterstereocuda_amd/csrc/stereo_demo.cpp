@@ -1,10 +1,13 @@
-// stereo_demo: drives the StereoCuda drop-in class exactly as a caller of the
-// reference library would (Init, Match, Release), on raw files.
+// stereo_demo: drives the drop-in classes exactly as a caller of the
+// reference libraries would (Init, Match, Release), on raw files.
 //
 //   stereo_demo LEFT.u8 RIGHT.u8 W H MIN_D RANGE OUT_LEFT.f32 [OUT_RIGHT.f32]
 //               [--cs 0|1] [--p1 N] [--p2 N] [--uniq F] [--lr 0|1] [--lrthr F]
-//               [--median 0|1] [--repeat N]
+//               [--median 0|1] [--repeat N] [--layers N]
 //
+// Without --layers: StereoCuda (single level, the options above).  With
+// --layers N: FasterStereoCuda (N-level pyramid over [MIN_D, MIN_D + RANGE),
+// its own fixed per-level options; only --lr and --repeat apply).
 // Inputs are tightly packed uint8 W x H; outputs tightly packed float32.
 // Prints the average Match time over --repeat calls after the first.
 #include <chrono>
@@ -14,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "FasterStereoCuda.h"
 #include "StereoCuda.h"
 
 static bool read_file(const char* path, std::vector<uint8>& buf, size_t n) {
@@ -55,7 +59,7 @@ int main(int argc, char** argv) {
     opt.lr_threshold = 1.0f;
     opt.do_median_filter = true;
     opt.using_constant_p2 = true;
-    int repeat = 1;
+    int repeat = 1, layers = 0;
     for (; a + 1 < argc; a += 2) {
         const std::string k = argv[a];
         const char* v = argv[a + 1];
@@ -67,6 +71,7 @@ int main(int argc, char** argv) {
         else if (k == "--lrthr") opt.lr_threshold = (float)std::atof(v);
         else if (k == "--median") opt.do_median_filter = std::atoi(v) != 0;
         else if (k == "--repeat") repeat = std::atoi(v);
+        else if (k == "--layers") layers = std::atoi(v);
         else { std::fprintf(stderr, "unknown option %s\n", k.c_str()); return 2; }
     }
 
@@ -77,6 +82,31 @@ int main(int argc, char** argv) {
         return 2;
     }
     std::vector<float32> dl(npix), dr(npix);
+
+    if (layers > 0) {
+        FasterStereoCuda fs;
+        FasterStereoCuda::ste_opt1_t o;
+        o.width = W; o.height = H; o.min_disp = min_d; o.max_disp = min_d + range;
+        o.num_layers = layers; o.do_lr_check = opt.do_lr_check;
+        o.do_rm_peaks = false; o.do_smooth = false;
+        if (!fs.Init(o)) {
+            std::fprintf(stderr, "Init failed: %s\n", fs.LastError());
+            return 1;
+        }
+        double ms = 0.0;
+        for (int r = 0; r < repeat; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if (!fs.Match(left.data(), right.data(), dl.data())) {
+                std::fprintf(stderr, "Match failed: %s\n", fs.LastError());
+                return 1;
+            }
+            const auto t1 = std::chrono::steady_clock::now();
+            if (r > 0 || repeat == 1) ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        }
+        fs.Release();
+        std::printf("match_ms %.4f\n", ms / (repeat > 1 ? repeat - 1 : 1));
+        return write_file(out_left, dl.data(), npix * 4) ? 0 : 1;
+    }
 
     StereoCuda stereo;
     if (!stereo.Init(W, H, min_d, range, opt, false)) {

@@ -1,6 +1,7 @@
 """The C ABI boundary, checked on the CPU (no compute calls):
   - libfsgm.so loads and exports every function include/sgm_capi.h declares
   - libStereoCuda.so exports every StereoCuda member include/StereoCuda.h declares
+  - libFasterStereoCuda.so exports every FasterStereoCuda member of its header
   - the ctypes mirror of sgm_option has the header's field order and types
   - argument validation happens before the device is touched
 """
@@ -52,6 +53,25 @@ def test_stereocuda_class_exports():
     for m in sorted(members):
         assert f"StereoCuda::{m}(" in exported, f"StereoCuda::{m} not exported"
     assert "StereoCuda::StereoCuda()" in exported and "StereoCuda::~StereoCuda()" in exported
+
+
+def test_faster_stereo_cuda_class_exports():
+    text = open(os.path.join(INCLUDE, "FasterStereoCuda.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    members = set(re.findall(r"\b(Init2?|Match2?|Release|LastError|MallocPageLockedPtr|"
+                             r"FreePageLockedPtr)\s*\(", text))
+    assert members == {"Init", "Init2", "Match", "Match2", "Release", "LastError",
+                       "MallocPageLockedPtr", "FreePageLockedPtr"}
+    exported = _exports(os.path.join(PKG, "libFasterStereoCuda.so"), demangle=True)
+    for m in sorted(members):
+        assert f"FasterStereoCuda::{m}(" in exported, f"FasterStereoCuda::{m} not exported"
+    assert "FasterStereoCuda::FasterStereoCuda()" in exported
+    # the option struct keeps the reference's field order and defaults
+    body = re.search(r"typedef struct StereoOption1 \{(.*?)\} ste_opt1_t;", text, re.S).group(1)
+    assert re.findall(r"(?:int|bool)\s+(\w+);", body) == [
+        "width", "height", "min_disp", "max_disp", "num_layers", "do_lr_check", "do_rm_peaks",
+        "do_smooth"]
+    assert "num_layers(1), do_lr_check(true)" in body.replace("\n", " ").replace("  ", " ")
 
 
 def test_option_struct_matches_header():

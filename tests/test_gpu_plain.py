@@ -152,6 +152,24 @@ def test_cpp_dropin_demo(cuda, tmp_path):
     _eq_float(np.fromfile(orr, np.float32).reshape(H, W), exp_r, "demo right")
 
 
+@pytest.mark.parametrize("layers", [1, 2, 3])
+def test_cpp_faster_stereo_cuda_demo(cuda, tmp_path, layers):
+    """The C++ FasterStereoCuda class (libFasterStereoCuda.so) through
+    stereo_demo --layers: the reference's pyramid schedule, against the oracle."""
+    demo = os.path.join(os.path.dirname(oracle.__file__), "..", "fasterstereocuda_amd", "stereo_demo")
+    pair = scenes.cone_like(3, H=300, W=400, D=128, workload=2)
+    H, W = pair.left.shape
+    lp, rp, ol = tmp_path / "l.u8", tmp_path / "r.u8", tmp_path / "d.f32"
+    pair.left.tofile(lp)
+    pair.right.tofile(rp)
+    print(f"[demo] FasterStereoCuda 400x300 D128 layers={layers}", flush=True)
+    res = subprocess.run([demo, str(lp), str(rp), str(W), str(H), "0", "128", str(ol),
+                          "--layers", str(layers)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr + res.stdout
+    exp = oracle.pyramid(pair.left, pair.right, 0, 128, layers, 32, cs_type=0, p1=8, p2=32)
+    _eq_float(np.fromfile(ol, np.float32).reshape(H, W), exp, "FasterStereoCuda demo")
+
+
 def test_errors_are_reported(cuda):
     st = StereoCuda()
     assert not st.Init(100, 50, 0, 96, CuSGMOption.benchmark())
