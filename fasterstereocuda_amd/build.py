@@ -19,8 +19,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["sgm_kernels.hip", "sgm_capi.hip", "dsgm_kernels.hip", "dsgm_capi.hip",
-               "pyr_kernels.hip", "pyr_capi.hip"]
+HIP_SOURCES = ["sgm_kernels.hip", "sgm_wide.hip", "sgm_capi.hip", "dsgm_kernels.hip",
+               "dsgm_capi.hip", "pyr_kernels.hip", "pyr_capi.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

@@ -7,6 +7,7 @@
 // check + median.  The reference runs the right view after the left one on
 // a shared cost buffer; here both views are jobs of the same launches.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -21,6 +22,7 @@
 #include "sgm_internal.h"
 
 using namespace fsgm;
+
 
 namespace {
 
@@ -43,6 +45,11 @@ int fail(std::string* dst, int code, const std::string& msg) {
 
 namespace fsgm {
 int set_global_error(int code, const std::string& msg) { return fail(nullptr, code, msg); }
+
+namespace {
+std::atomic<int> g_cost_mode{0};
+}  // namespace
+int cost_mode() { return g_cost_mode.load(); }
 }  // namespace fsgm
 
 namespace {
@@ -76,9 +83,15 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
                 StagePlan* plan, void** block, std::string* err) {
     const int nj = (int)jobs.size(), np = (int)posts.size();
     std::vector<int> oc(nj + 1, 0), ok(nj + 1, 0), oa(nj + 1, 0), ow(nj + 1, 0), op(np + 1, 0);
-    std::vector<int> o3(nj + 1, 0), od(nj + 1, 0);
+    std::vector<int> o3(nj + 1, 0), od(nj + 1, 0), w3(nj + 1, 0), wd(nj + 1, 0), wf(nj + 1, 0);
     int dmax = 64;
+    bool wide = cost_mode() == 0;
     for (int i = 0; i < nj; ++i) {
+        wide = wide && wide_ok(jobs[i]);
+        const bool w = jobs[i].D <= 256;
+        w3[i + 1] = w3[i] + (w ? wide_h_items(jobs[i]) : 0);
+        wd[i + 1] = wd[i] + (w ? wide_v_items(jobs[i], kWideV) : 0);
+        wf[i + 1] = wf[i] + (w ? wide_v_items(jobs[i], kWideF) : 0);
         oc[i + 1] = oc[i] + census_items(jobs[i]);
         ok[i + 1] = ok[i] + cost_items(jobs[i]);
         oa[i + 1] = oa[i] + agg_items(jobs[i]);
@@ -92,7 +105,7 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     const size_t s_posts = align256(sizeof(SgmPost) * std::max(np, 1));
     const size_t s_oj = align256(sizeof(int) * (nj + 1));
     const size_t s_op = align256(sizeof(int) * (np + 1));
-    const size_t total = s_jobs + s_posts + 6 * s_oj + s_op;
+    const size_t total = s_jobs + s_posts + 9 * s_oj + s_op;
     std::vector<uint8_t> host(total, 0);
     size_t off = 0;
     auto put = [&](const void* src, size_t n, size_t slot) {
@@ -109,6 +122,9 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     const size_t a_ow = put(ow.data(), sizeof(int) * (nj + 1), s_oj);
     const size_t a_o3 = put(o3.data(), sizeof(int) * (nj + 1), s_oj);
     const size_t a_od = put(od.data(), sizeof(int) * (nj + 1), s_oj);
+    const size_t a_w3 = put(w3.data(), sizeof(int) * (nj + 1), s_oj);
+    const size_t a_wd = put(wd.data(), sizeof(int) * (nj + 1), s_oj);
+    const size_t a_wf = put(wf.data(), sizeof(int) * (nj + 1), s_oj);
     const size_t a_op = put(op.data(), sizeof(int) * (np + 1), s_op);
     if (*block) {
         (void)hipFree(*block);
@@ -125,6 +141,9 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     plan->offs_wta = reinterpret_cast<int*>(b + a_ow);
     plan->offs_agg3 = reinterpret_cast<int*>(b + a_o3);
     plan->offs_dw = reinterpret_cast<int*>(b + a_od);
+    plan->offs_wh = reinterpret_cast<int*>(b + a_w3);
+    plan->offs_wv = reinterpret_cast<int*>(b + a_wd);
+    plan->offs_wf = reinterpret_cast<int*>(b + a_wf);
     plan->offs_post = reinterpret_cast<int*>(b + a_op);
     plan->njobs = nj;
     plan->nposts = np;
@@ -134,6 +153,10 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     plan->n_wta = ow[nj];
     plan->n_agg3 = o3[nj];
     plan->n_dw = od[nj];
+    plan->n_wh = w3[nj];
+    plan->n_wv = wd[nj];
+    plan->n_wf = wf[nj];
+    plan->wide = wide;
     plan->n_post = op[np];
     plan->dmax = dmax;
     return SGM_OK;
@@ -143,14 +166,22 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
 // offsets come first, so find_job over n jobs stays valid.
 void fsgm::restrict_matching(StagePlan* P, const std::vector<SgmJob>& jobs, int n) {
     P->njobs_ctw = n;
-    P->n_cost = P->n_agg = P->n_wta = P->n_agg3 = P->n_dw = 0;
+    P->n_cost = P->n_agg = P->n_wta = P->n_agg3 = P->n_dw = P->n_wh = P->n_wv = P->n_wf = 0;
+    bool wide = cost_mode() == 0;
     for (int i = 0; i < n; ++i) {
+        wide = wide && wide_ok(jobs[i]);
+        if (jobs[i].D <= 256) {
+            P->n_wh += wide_h_items(jobs[i]);
+            P->n_wv += wide_v_items(jobs[i], kWideV);
+            P->n_wf += wide_v_items(jobs[i], kWideF);
+        }
         P->n_cost += cost_items(jobs[i]);
         P->n_agg += agg_items(jobs[i]);
         P->n_wta += wta_items(jobs[i]);
         P->n_agg3 += agg3_items(jobs[i]);
         P->n_dw += dw_items(jobs[i]);
     }
+    P->wide = wide;
 }
 
 // census, cost, three directions, bottom->top + WTA, post, [right-map LR
@@ -165,14 +196,20 @@ hipError_t fsgm::launch_match(const StagePlan& plan, bool right_invalidation, hi
     if ((e = mark(0)) != hipSuccess) return e;
     if ((e = launch_census(plan, s)) != hipSuccess) return e;
     if ((e = mark(1)) != hipSuccess) return e;
-    if ((e = launch_cost(ctw, s)) != hipSuccess) return e;
+    // with the on-the-fly cost there is no cost volume to build
+    if (!plan.wide && (e = launch_cost(ctw, s)) != hipSuccess) return e;
     if ((e = mark(2)) != hipSuccess) return e;
-    if ((e = sched ? launch_aggregate_sched(ctw, *sched, s) : launch_aggregate3(ctw, s)) != hipSuccess)
-        return e;
+    if (plan.wide)
+        e = sched ? launch_wide3_sched(ctw, *sched, s) : launch_wide3(ctw, s);
+    else
+        e = sched ? launch_aggregate_sched(ctw, *sched, s) : launch_aggregate3(ctw, s);
+    if (e != hipSuccess) return e;
     if ((e = mark(3)) != hipSuccess) return e;
-    if ((e = sched ? launch_aggregate_wta_sched(ctw, *sched, s) : launch_aggregate_wta(ctw, s)) !=
-        hipSuccess)
-        return e;
+    if (plan.wide)
+        e = sched ? launch_wide_wta_sched(ctw, *sched, s) : launch_wide_wta(ctw, s);
+    else
+        e = sched ? launch_aggregate_wta_sched(ctw, *sched, s) : launch_aggregate_wta(ctw, s);
+    if (e != hipSuccess) return e;
     if ((e = mark(4)) != hipSuccess) return e;
     if ((e = launch_post(plan, s)) != hipSuccess) return e;
     if (right_invalidation && (e = launch_right_invalidate(plan, s)) != hipSuccess) return e;
@@ -444,9 +481,20 @@ void sgm_roi_from_disparity(const float* disp, int32_t width, int32_t height, sg
             }
 }
 
+int sgm_set_cost_mode(int32_t mode) {
+    if (mode != SGM_COST_AUTO && mode != SGM_COST_STORED)
+        return fail(nullptr, SGM_E_INVALID_ARG, "cost mode must be SGM_COST_AUTO or SGM_COST_STORED");
+    fsgm::g_cost_mode.store(mode);
+    return SGM_OK;
+}
+
+int32_t sgm_get_cost_mode(void) { return fsgm::cost_mode(); }
+
 float sgm_invalid_float(void) { return SGM_INVALID_FLOAT; }
 int16_t sgm_invalid_short(void) { return SGM_INVALID_SHORT; }
 int16_t sgm_level_range(void) { return SGM_LEVEL_RANGE; }
+
+}  // extern "C"
 
 // --------------------------------------------------------------- batches
 struct sgm_batch {
@@ -517,31 +565,41 @@ std::string batch_signature(int device, const sgm_option& o, const sgm_pair_desc
 // Packed aggregation items (job << 16 | direction << 14 | group) of the
 // matched jobs, in the order `policy` asks for.  Directions 0..3 are
 // top->bottom, left->right, bottom->top, right->left.
-void build_items(const std::vector<SgmJob>& jobs, int policy, std::vector<uint32_t>* agg3,
-                 std::vector<uint32_t>* dw) {
+int build_items(const std::vector<SgmJob>& jobs, bool wide, int policy, std::vector<uint32_t>* agg3,
+                std::vector<uint32_t>* dw) {
     struct Item { int64_t work; uint32_t code; };
-    std::vector<Item> a, b;
+    std::vector<Item> a, v, b;   // horizontal, top->bottom, bottom->top
     for (size_t j = 0; j < jobs.size(); ++j) {
         const SgmJob& J = jobs[j];
-        const int nh = (J.H + 3) / 4, wv = J.W - J.roi_x, nv = (wv + 3) / 4;
+        // scanlines per group of each kernel
+        const int lh = wide ? wide_lines(kWideH, J.D) : 4, lv = wide ? wide_lines(kWideV, J.D) : 4;
+        const int lf = wide ? wide_lines(kWideF, J.D) : 4;
+        const int wv = J.W - J.roi_x;
         auto code = [&](int dir, int g) { return (uint32_t)j << 16 | (uint32_t)dir << 14 | (uint32_t)g; };
         for (int dir : {1, 3})   // left->right, right->left
-            for (int g = 0; g < nh; ++g)
-                a.push_back({(int64_t)std::min(4, J.H - 4 * g) * wv * J.D, code(dir, g)});
-        for (int g = 0; g < nv; ++g)
-            a.push_back({(int64_t)std::min(4, wv - 4 * g) * J.H * J.D, code(0, g)});
-        for (int g = 0; g < nv; ++g)
-            b.push_back({(int64_t)std::min(4, wv - 4 * g) * J.H * J.D, code(2, g)});
+            for (int g = 0; g < (J.H + lh - 1) / lh; ++g)
+                a.push_back({(int64_t)std::min(lh, J.H - lh * g) * wv * J.D, code(dir, g)});
+        for (int g = 0; g < (wv + lv - 1) / lv; ++g)
+            v.push_back({(int64_t)std::min(lv, wv - lv * g) * J.H * J.D, code(0, g)});
+        for (int g = 0; g < (wv + lf - 1) / lf; ++g)
+            b.push_back({(int64_t)std::min(lf, wv - lf * g) * J.H * J.D, code(2, g)});
+    }
+    if (!wide) {   // one kernel takes every direction: a single list
+        a.insert(a.end(), v.begin(), v.end());
+        v.clear();
     }
     if (policy == SGM_SCHED_LONGEST_FIRST) {
         auto by_work = [](const Item& x, const Item& y) { return x.work > y.work; };
         std::stable_sort(a.begin(), a.end(), by_work);
+        std::stable_sort(v.begin(), v.end(), by_work);
         std::stable_sort(b.begin(), b.end(), by_work);
     }
     agg3->clear();
     dw->clear();
     for (const Item& x : a) agg3->push_back(x.code);
+    for (const Item& x : v) agg3->push_back(x.code);
     for (const Item& x : b) dw->push_back(x.code);
+    return (int)a.size();
 }
 
 void drop_graph(sgm_batch* b) {
@@ -566,12 +624,13 @@ int apply_schedule(sgm_batch* b, int policy, int waves) {
         if (J.H > 65532 || J.W > 65532)
             return fail(&b->err, SGM_E_INVALID_ARG, "persistent schedules take sides up to 65532");
     std::vector<uint32_t> a3, dw;
-    build_items(b->jobs, policy, &a3, &dw);
-    const size_t bytes = (a3.size() + dw.size() + 2) * sizeof(uint32_t);
+    const int n3h = build_items(b->jobs, b->plan.wide, policy, &a3, &dw);
+    const size_t bytes = (a3.size() + dw.size() + 3) * sizeof(uint32_t);
     HIP_TRY(&b->err, hipSetDevice(b->device));
     if (!b->sched_block) HIP_TRY(&b->err, hipMalloc(&b->sched_block, bytes));
     std::vector<uint32_t> host(a3);
     host.insert(host.end(), dw.begin(), dw.end());
+    host.push_back(0u);
     host.push_back(0u);
     host.push_back(0u);
     // earlier runs may still read the old list
@@ -580,6 +639,7 @@ int apply_schedule(sgm_batch* b, int policy, int waves) {
     uint32_t* base = static_cast<uint32_t*>(b->sched_block);
     b->sched.items3 = base;
     b->sched.n3 = (int)a3.size();
+    b->sched.n3h = n3h;
     b->sched.items_dw = base + a3.size();
     b->sched.ndw = (int)dw.size();
     b->sched.counters = base + a3.size() + dw.size();
@@ -713,7 +773,8 @@ sgm_batch* sgm_batch_create(const sgm_pair_desc* pairs, int32_t n, const sgm_opt
     b->jobs.assign(jobs.begin(), jobs.begin() + (lr ? jobs.size() : (size_t)n));
     b->signature = batch_signature(b->device, *opt, pairs, n, b->want_right);
     if (hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
-        persistent_occupancy(b->plan, &b->per_cu) != hipSuccess) {
+        (b->plan.wide ? wide_occupancy(b->plan, &b->per_cu)
+                      : persistent_occupancy(b->plan, &b->per_cu)) != hipSuccess) {
         fail(nullptr, SGM_E_HIP, "device query failed");
         free_batch(b);
         return nullptr;
@@ -906,6 +967,8 @@ int run_stage(const SgmJob& j, const SgmPost* post, hipError_t (*fn)(const Stage
 }
 
 }  // namespace
+
+extern "C" {
 
 int sgm_stage_census(const uint8_t* d_img, int32_t width, int32_t height, int32_t cs_type,
                      int32_t is_right, uint64_t* d_census, void* stream) {

@@ -57,6 +57,35 @@ inline int agg3_items(const SgmJob& j) { return 2 * ((j.H + 3) / 4) + ((j.W - j.
 inline int dw_items(const SgmJob& j) { return (j.W - j.roi_x + 3) / 4; }
 inline int wta_items(const SgmJob& j) { return j.H * ((j.W + 15) / 16); }
 inline int post_items(const SgmPost& p) { return ((p.W + 15) / 16) * ((p.H + 15) / 16); }
+// On-the-fly cost aggregation (sgm_wide.hip).  Each kernel holds DPL
+// disparities per lane, so a scanline takes D / DPL lanes (at most one
+// 16-lane DPP row) and a wave covers 64 * DPL / D scanlines.
+enum WideKind { kWideH = 0, kWideV = 1, kWideF = 2 };   // LR + RL, UD, DU + WTA
+#ifndef FSGM_DPL_H
+#define FSGM_DPL_H 16
+#endif
+#ifndef FSGM_DPL_V
+#define FSGM_DPL_V 8
+#endif
+#ifndef FSGM_DPL_F
+#define FSGM_DPL_F 8
+#endif
+__host__ __device__ constexpr int wide_dpl(int kind, int D) {
+    const int want = kind == kWideH ? FSGM_DPL_H : kind == kWideV ? FSGM_DPL_V : FSGM_DPL_F;
+    return want * 16 >= D ? want : D / 16;   // D / DPL <= 16
+}
+__host__ __device__ constexpr int wide_lines(int kind, int D) { return 64 * wide_dpl(kind, D) / D; }
+inline int wide_h_items(const SgmJob& j) {
+    const int l = wide_lines(kWideH, j.D);
+    return 2 * ((j.H + l - 1) / l);
+}
+inline int wide_v_items(const SgmJob& j, int kind) {
+    const int l = wide_lines(kind, j.D);
+    return (j.W - j.roi_x + l - 1) / l;
+}
+// Whether a job's cost can come from the census on the fly: D <= 256, and not
+// the 5x5 right view, whose skipped-pixel rule needs the stored cost.
+inline bool wide_ok(const SgmJob& j) { return j.D <= 256 && !(j.is_right && j.cs_type == 0); }
 
 // Device-side launch plan: jobs, per-stage prefix sums and totals.
 struct StagePlan {
@@ -67,12 +96,17 @@ struct StagePlan {
     int* offs_wta = nullptr;
     int* offs_agg3 = nullptr;     // three directions
     int* offs_dw = nullptr;       // bottom->top fused with the WTA
+    int* offs_wh = nullptr;       // on-the-fly cost: left->right + right->left groups
+    int* offs_wv = nullptr;       // on-the-fly cost: top->bottom groups
+    int* offs_wf = nullptr;       // on-the-fly cost: bottom->top + WTA groups
     SgmPost* posts = nullptr;     // device
     int* offs_post = nullptr;
     int njobs = 0, nposts = 0;
     int njobs_ctw = 0;            // if > 0: jobs [0, njobs_ctw) run cost..wta
     int n_census = 0, n_cost = 0, n_agg = 0, n_wta = 0, n_post = 0, n_agg3 = 0, n_dw = 0;
+    int n_wh = 0, n_wv = 0, n_wf = 0;
     int dmax = 64;
+    bool wide = false;            // matching uses the on-the-fly cost kernels
 };
 
 // Scanline schedule of the persistent aggregation launches: packed items
@@ -80,9 +114,10 @@ struct StagePlan {
 struct AggSchedule {
     const uint32_t* items3 = nullptr;   // LR, RL, UD items
     int n3 = 0;
+    int n3h = 0;                        // on-the-fly plans: items3 starts with n3h LR / RL items
     const uint32_t* items_dw = nullptr; // bottom->top + WTA items
     int ndw = 0;
-    uint32_t* counters = nullptr;       // [2], reset by the first launch
+    uint32_t* counters = nullptr;       // [3], reset by the first launch
     int grid = 0;                       // resident waves
 };
 
@@ -96,6 +131,16 @@ hipError_t launch_aggregate_sched(const StagePlan& p, const AggSchedule& sc, hip
 hipError_t launch_aggregate_wta_sched(const StagePlan& p, const AggSchedule& sc, hipStream_t s);
 // Resident 64-lane workgroups per compute unit of the persistent kernels.
 hipError_t persistent_occupancy(const StagePlan& p, int* per_cu);
+hipError_t launch_reset_counters(uint32_t* counters, int n, hipStream_t s);
+// sgm_wide.hip: aggregation with the cost computed from the census words
+hipError_t launch_wide3(const StagePlan& p, hipStream_t s);
+hipError_t launch_wide_wta(const StagePlan& p, hipStream_t s);
+hipError_t launch_wide3_sched(const StagePlan& p, const AggSchedule& sc, hipStream_t s);
+hipError_t launch_wide_wta_sched(const StagePlan& p, const AggSchedule& sc, hipStream_t s);
+hipError_t wide_occupancy(const StagePlan& p, int* per_cu);
+// Process-wide: 0 = on-the-fly cost wherever a plan allows it, 1 = always the
+// stored cost volume.  Read when a plan is built.
+int cost_mode();
 hipError_t launch_wta(const StagePlan& p, hipStream_t s);
 hipError_t launch_post(const StagePlan& p, hipStream_t s);
 hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s);

@@ -56,6 +56,21 @@ class CuSGMOption:
         return o
 
 
+COST_AUTO, COST_STORED = 0, 1
+
+
+def set_cost_mode(mode: int) -> None:
+    """Where the aggregation takes its matching cost, for handles, batches
+    and pyramids created afterwards: COST_AUTO recomputes it from the census
+    words on the fly where possible, COST_STORED always builds the u8 cost
+    volume.  Results are byte-identical."""
+    check(lib().sgm_set_cost_mode(int(mode)))
+
+
+def get_cost_mode() -> int:
+    return int(lib().sgm_get_cost_mode())
+
+
 class StereoCuda:
     """Single-level SGM matcher (StereoCuda.h)."""
 

@@ -103,6 +103,19 @@ int sgm_host_free(void* ptr);
  * StereoCudaImpl.cpp:585-628): host-side scan. */
 void sgm_roi_from_disparity(const float* disp, int32_t width, int32_t height, sgm_roi* roi);
 
+/* Where the aggregation takes its matching cost (no reference counterpart;
+ * the reference always stores the cost volume, cusgm_cc.cu:118-172):
+ *   SGM_COST_AUTO    recompute it from the census words inside the
+ *                    aggregation wherever the plan allows (D <= 256, not the
+ *                    5x5 right view): no cost volume is built or read
+ *   SGM_COST_STORED  always build and read the u8 cost volume
+ * Process-wide; read when a handle, batch or pyramid is created.  Both give
+ * byte-identical results. */
+#define SGM_COST_AUTO 0
+#define SGM_COST_STORED 1
+int sgm_set_cost_mode(int32_t mode);
+int32_t sgm_get_cost_mode(void);
+
 /* StereoCuda::get_invad_float / get_invad_short / get_level_range
  * (StereoCuda.h:87-91). */
 float sgm_invalid_float(void);

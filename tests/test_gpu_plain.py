@@ -113,6 +113,38 @@ def test_stereocuda_match_matches_oracle(cuda, case, lr, median):
     st.Release()
 
 
+@pytest.fixture
+def cost_mode():
+    from fasterstereocuda_amd import stereo
+    before = stereo.get_cost_mode()
+    yield stereo.set_cost_mode
+    stereo.set_cost_mode(before)
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["on-the-fly", "stored"])
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+@pytest.mark.parametrize("lr", [True, False])
+def test_cost_modes_match_oracle(cuda, cost_mode, mode, case, lr):
+    """The on-the-fly cost kernels (census words read inside the aggregation,
+    no cost volume) and the stored-cost kernels give the oracle's maps, for
+    every D, both census sizes, +-min_d, P1 > 127 and ragged sizes."""
+    W, H, D, min_d, cs, p1, p2 = case
+    print(f"[cost mode {mode}] {case} lr={lr}", flush=True)
+    cost_mode(mode)
+    left, right = _pair(W, H, D, seed=5)
+    exp_l, exp_r = oracle.match(left, right, min_d, D, cs_type=cs, p1=p1, p2=p2, uniqueness=UNIQ,
+                                do_lr_check=lr, lr_threshold=LR_THR, do_median=True)
+    st = StereoCuda()
+    opt = CuSGMOption.benchmark(cs_type=cs, p1=p1, p2_init=p2, do_lr_check=lr)
+    assert st.Init(W, H, min_d, D, opt), st.last_error
+    dl = np.empty((H, W), np.float32)
+    dr = np.empty((H, W), np.float32)
+    assert st.Match(left, right, dl, dr), st.last_error
+    _eq_float(dl, exp_l, "left map")
+    _eq_float(dr, exp_r, "right map")
+    st.Release()
+
+
 def test_driving_full_size(cuda):
     pair = scenes.driving(0)
     print("[match] driving 1242x375", flush=True)
