@@ -116,12 +116,17 @@ def main():
     agg_bytes = AGG_BYTES_PER_CELL * cells_per_pair * NP * 2      # both views
     achieved = agg_bytes / (agg_ms * 1e-3) / 1e9
 
+    # which aggregation kernels ran: the cost from the census on the fly
+    # (D <= 256, 9x7 census) or the stored cost volume
+    from fasterstereocuda_amd.stereo import get_cost_mode
+    path = "on-the-fly" if (get_cost_mode() == 0 and D <= 256) else "stored"
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "aggregate_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             t = json.load(f)
-        if (t.get("pairs"), t.get("width"), t.get("height"), t.get("disp_range")) == (NP, W, H, D):
+        if (t.get("pairs"), t.get("width"), t.get("height"), t.get("disp_range"),
+                t.get("path")) == (NP, W, H, D, path):
             traffic = t.get("bytes_per_launch")
 
     # plain path: one StereoCuda Match per pair on the same device buffers
@@ -213,7 +218,10 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "aggregate_kernel<1> + aggregate_kernel<2> (bottom->top fused with WTA)",
+            "kernel": ("wide_h_kernel + wide_v_kernel<false> + wide_v_kernel<true> (bottom->top "
+                       "fused with WTA), cost from the census on the fly" if path == "on-the-fly" else
+                       "aggregate_kernel<1> + aggregate_kernel<2> (bottom->top fused with WTA), "
+                       "stored cost volume"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -65,7 +65,7 @@ enum WideKind { kWideH = 0, kWideV = 1, kWideF = 2 };   // LR + RL, UD, DU + WTA
 #define FSGM_DPL_H 16
 #endif
 #ifndef FSGM_DPL_V
-#define FSGM_DPL_V 8
+#define FSGM_DPL_V 16
 #endif
 #ifndef FSGM_DPL_F
 #define FSGM_DPL_F 8

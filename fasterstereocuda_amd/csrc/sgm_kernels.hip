@@ -32,16 +32,37 @@ using namespace dev;
 // ---------------------------------------------------------------- census ---
 // cusgm_cc.cu:30-106.  Bit = (neighbour < centre), MSB-first raster order,
 // centre included; border pixels get 0 (left view) or 0xFFFFFFFF (right).
+// The thread's 4 vertically adjacent pixels share the window rows they
+// overlap in: 4 + 2 RH tile rows of 2 RW + 1 bytes are read once.  A bit is
+// two VALU ops: d = neighbour - centre (as u32, negative iff neighbour <
+// centre) and acc = alignbit(acc, d, 31) = (acc << 1) | (d >> 31).  The first
+// NB - 32 bits (NB = window size) go to the high word, the last 32 to the low.
 template <int RW, int RH>
-__device__ __forceinline__ uint64_t census_px(const uint8_t (*t)[72], int r, int c) {
-    const uint8_t ctr = t[r][c];
-    uint64_t v = 0;
+__device__ __forceinline__ void census_column4(const uint8_t (*t)[72], int r0, int c,
+                                               uint64_t (&out)[4]) {
+    constexpr int NJ = 2 * RW + 1, NB = NJ * (2 * RH + 1);
+    uint32_t hi[4] = {0u, 0u, 0u, 0u}, lo[4] = {0u, 0u, 0u, 0u}, ctr[4];
 #pragma unroll
-    for (int i = -RH; i <= RH; ++i)
+    for (int p = 0; p < 4; ++p) ctr[p] = t[r0 + p][c];
 #pragma unroll
-        for (int j = -RW; j <= RW; ++j)
-            v = (v << 1) | (uint64_t)(t[r + i][c + j] < ctr);
-    return v;
+    for (int k = 0; k < 4 + 2 * RH; ++k) {      // tile rows r0 - RH .. r0 + 3 + RH
+        uint32_t nb[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) nb[j] = t[r0 - RH + k][c - RW + j];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int i = k - p;                 // window row of pixel p
+            if (i < 0 || i > 2 * RH) continue;
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                const uint32_t d = nb[j] - ctr[p];
+                if (i * NJ + j < NB - 32) hi[p] = __builtin_amdgcn_alignbit(hi[p], d, 31);
+                else lo[p] = __builtin_amdgcn_alignbit(lo[p], d, 31);
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) out[p] = ((uint64_t)hi[p] << 32) | lo[p];
 }
 
 __global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ jobs,
@@ -64,18 +85,15 @@ __global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ 
     const int lx = tid & 63, ly = tid >> 6;
     const int rw = J.cs_type ? 4 : 2, rh = J.cs_type ? 3 : 2;
     const uint64_t invalid = J.is_right ? 0xFFFFFFFFull : 0ull;
+    uint64_t v[4];
+    if (J.cs_type) census_column4<4, 3>(tile, ly * 4 + 3, lx + 4, v);
+    else census_column4<2, 2>(tile, ly * 4 + 3, lx + 4, v);
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
         const int y = y0 + ly * 4 + rr, x = x0 + lx;
         if (x >= J.W || y >= J.H) continue;
-        uint64_t v;
-        if (x < rw || x >= J.W - rw || y < rh || y >= J.H - rh)
-            v = invalid;
-        else if (J.cs_type)
-            v = census_px<4, 3>(tile, ly * 4 + rr + 3, lx + 4);
-        else
-            v = census_px<2, 2>(tile, ly * 4 + rr + 3, lx + 4);
-        gst(J.census + (int64_t)y * J.W + x, v);
+        const bool border = x < rw || x >= J.W - rw || y < rh || y >= J.H - rh;
+        gst(J.census + (int64_t)y * J.W + x, border ? invalid : v[rr]);
     }
 }
 
