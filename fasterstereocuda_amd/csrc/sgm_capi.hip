@@ -48,8 +48,32 @@ int set_global_error(int code, const std::string& msg) { return fail(nullptr, co
 
 namespace {
 std::atomic<int> g_cost_mode{0};
+std::atomic<int> g_wide_min_groups{-1};
 }  // namespace
 int cost_mode() { return g_cost_mode.load(); }
+
+// Horizontal scanline groups a plan needs before SGM_COST_AUTO picks the
+// on-the-fly kernels: two waves per SIMD of the current device.  With fewer,
+// their long scanlines (1024 / D lines per wave) leave the GPU idle and the
+// stored-cost kernels (4 lines per wave) finish first.
+int wide_min_groups() {
+    int v = g_wide_min_groups.load();
+    if (v < 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
+        v = 8 * cus;
+        g_wide_min_groups.store(v);
+    }
+    return v;
+}
+
+bool choose_wide(bool eligible, int n_wh) {
+    const int mode = cost_mode();
+    if (!eligible || mode == SGM_COST_STORED) return false;
+    return mode == SGM_COST_ON_THE_FLY || n_wh >= wide_min_groups();
+}
 }  // namespace fsgm
 
 namespace {
@@ -85,7 +109,7 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     std::vector<int> oc(nj + 1, 0), ok(nj + 1, 0), oa(nj + 1, 0), ow(nj + 1, 0), op(np + 1, 0);
     std::vector<int> o3(nj + 1, 0), od(nj + 1, 0), w3(nj + 1, 0), wd(nj + 1, 0), wf(nj + 1, 0);
     int dmax = 64;
-    bool wide = cost_mode() == 0;
+    bool wide = true;
     for (int i = 0; i < nj; ++i) {
         wide = wide && wide_ok(jobs[i]);
         const bool w = jobs[i].D <= 256;
@@ -156,7 +180,7 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     plan->n_wh = w3[nj];
     plan->n_wv = wd[nj];
     plan->n_wf = wf[nj];
-    plan->wide = wide;
+    plan->wide = choose_wide(wide, w3[nj]);
     plan->n_post = op[np];
     plan->dmax = dmax;
     return SGM_OK;
@@ -167,7 +191,7 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
 void fsgm::restrict_matching(StagePlan* P, const std::vector<SgmJob>& jobs, int n) {
     P->njobs_ctw = n;
     P->n_cost = P->n_agg = P->n_wta = P->n_agg3 = P->n_dw = P->n_wh = P->n_wv = P->n_wf = 0;
-    bool wide = cost_mode() == 0;
+    bool wide = true;
     for (int i = 0; i < n; ++i) {
         wide = wide && wide_ok(jobs[i]);
         if (jobs[i].D <= 256) {
@@ -181,7 +205,7 @@ void fsgm::restrict_matching(StagePlan* P, const std::vector<SgmJob>& jobs, int 
         P->n_agg3 += agg3_items(jobs[i]);
         P->n_dw += dw_items(jobs[i]);
     }
-    P->wide = wide;
+    P->wide = choose_wide(wide, P->n_wh);
 }
 
 // census, cost, three directions, bottom->top + WTA, post, [right-map LR
@@ -482,8 +506,9 @@ void sgm_roi_from_disparity(const float* disp, int32_t width, int32_t height, sg
 }
 
 int sgm_set_cost_mode(int32_t mode) {
-    if (mode != SGM_COST_AUTO && mode != SGM_COST_STORED)
-        return fail(nullptr, SGM_E_INVALID_ARG, "cost mode must be SGM_COST_AUTO or SGM_COST_STORED");
+    if (mode != SGM_COST_AUTO && mode != SGM_COST_STORED && mode != SGM_COST_ON_THE_FLY)
+        return fail(nullptr, SGM_E_INVALID_ARG,
+                    "cost mode must be SGM_COST_AUTO, SGM_COST_STORED or SGM_COST_ON_THE_FLY");
     fsgm::g_cost_mode.store(mode);
     return SGM_OK;
 }

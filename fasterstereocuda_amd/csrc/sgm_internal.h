@@ -14,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "sgm_capi.h"
+
 namespace fsgm {
 
 constexpr float kInvalidDisp = -999999.0f;
@@ -68,7 +70,7 @@ enum WideKind { kWideH = 0, kWideV = 1, kWideF = 2 };   // LR + RL, UD, DU + WTA
 #define FSGM_DPL_V 16
 #endif
 #ifndef FSGM_DPL_F
-#define FSGM_DPL_F 8
+#define FSGM_DPL_F 16
 #endif
 __host__ __device__ constexpr int wide_dpl(int kind, int D) {
     const int want = kind == kWideH ? FSGM_DPL_H : kind == kWideV ? FSGM_DPL_V : FSGM_DPL_F;
@@ -138,9 +140,11 @@ hipError_t launch_wide_wta(const StagePlan& p, hipStream_t s);
 hipError_t launch_wide3_sched(const StagePlan& p, const AggSchedule& sc, hipStream_t s);
 hipError_t launch_wide_wta_sched(const StagePlan& p, const AggSchedule& sc, hipStream_t s);
 hipError_t wide_occupancy(const StagePlan& p, int* per_cu);
-// Process-wide: 0 = on-the-fly cost wherever a plan allows it, 1 = always the
-// stored cost volume.  Read when a plan is built.
+// Process-wide SGM_COST_* mode, read when a plan is built; choose_wide
+// applies it to a plan whose views all allow the on-the-fly cost
+// (`eligible`) and that has n_wh horizontal scanline groups.
 int cost_mode();
+bool choose_wide(bool eligible, int n_wh);
 hipError_t launch_wta(const StagePlan& p, hipStream_t s);
 hipError_t launch_post(const StagePlan& p, hipStream_t s);
 hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s);

@@ -55,15 +55,6 @@ __device__ __forceinline__ uint32_t line_min(uint32_t v) {
     if constexpr (LPL >= 16) v = min(v, dpp<kRowMirror>(0u, v));
     return v;
 }
-template <int LPL>
-__device__ __forceinline__ uint32_t line_or(uint32_t v) {
-    v |= dpp<kQuadXor1>(0u, v);
-    v |= dpp<kQuadXor2>(0u, v);
-    if constexpr (LPL >= 8) v |= dpp<kRowHalfMirror>(0u, v);
-    if constexpr (LPL >= 16) v |= dpp<kRowMirror>(0u, v);
-    return v;
-}
-
 // value of the previous / next lane of the line; the line's first / last
 // lane gets `edge`
 template <int LPL>
@@ -292,23 +283,9 @@ struct VertLds {
                                             : VertGeom<64, wide_dpl(KIND, 64)>::NV;
 };
 
-// w = S[k] for a lane-varying k in [0, NP) (binary mux)
-template <int NP>
-__device__ __forceinline__ uint32_t select_pair(const uint32_t (&S)[NP], uint32_t k) {
-    uint32_t t[NP];
-#pragma unroll
-    for (int i = 0; i < NP; ++i) t[i] = S[i];
-#pragma unroll
-    for (int w = 1; w < NP; w *= 2) {
-        const bool hi = (k & (uint32_t)w) != 0u;
-#pragma unroll
-        for (int i = 0; i + w < NP; i += 2 * w) t[i] = hi ? t[i + w] : t[i];
-    }
-    return t[0];
-}
-
 template <int D, bool FUSE>
-__device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int lane, uint64_t* lds) {
+__device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int lane, uint64_t* lds,
+                                              uint16_t* sums /* FUSE: [lines][D] */) {
     constexpr int KIND = FUSE ? kWideF : kWideV;
     constexpr int DPL = wide_dpl(KIND, D), NP = DPL / 2, NW = NP / 2;
     using G = VertGeom<D, DPL>;
@@ -362,6 +339,7 @@ __device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int la
     // with those lanes finishing their pixels with the float rules.
     const float uthr = 1.0f - J.uniqueness;
     uint32_t kb = 0u, kboth = 0u;
+    uint16_t* const sline = sums + sub * D;   // this line's sums (FUSE)
     const uint8_t* srcs[3] = {J.path[kUD], J.path[kLR], J.path[kRL]};
     auto wta_flush = [&](int i0) __attribute__((always_inline)) {
         const int i = i0 + q;
@@ -406,12 +384,18 @@ __device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int la
         }
         best = line_min<LPL>(best);
         const int ds = (int)(best & 0xFFFFu);
-        // S(ds - 1) and S(ds + 1): one owner lane each, OR-reduced over the line
-        const uint32_t k1 = (uint32_t)(ds - 1 - DPL * q), k3 = k1 + 2u;
-        const uint32_t v1 = (k1 < (uint32_t)DPL) ? __builtin_amdgcn_ubfe(select_pair(S, k1 >> 1), k1 << 4, 16) : 0u;
-        const uint32_t v3 = (k3 < (uint32_t)DPL) ? __builtin_amdgcn_ubfe(select_pair(S, k3 >> 1), k3 << 4, 16) : 0u;
-        const uint32_t both = line_or<LPL>((v1 << 16) | v3);
-        if (q == ((i - 1) & 3)) { kb = best; kboth = both; }
+        // S(ds - 1) and S(ds + 1): the lanes of a line hold consecutive
+        // disparities, so their packed sums laid side by side in LDS are the
+        // line's u16 S[d]; every lane then reads the two it needs.  Indices
+        // are clamped: S(-1) and S(D) are never used (ds = 0 is invalid, and
+        // ds = D - 1 takes c3 = D - 1).
+        uint32_t* const mine = reinterpret_cast<uint32_t*>(sline + DPL * q);
+#pragma unroll
+        for (int k = 0; k < NP; ++k) mine[k] = S[k];
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t c1 = sline[max(ds - 1, 0)], c3 = sline[min(ds + 1, D - 1)];
+        __builtin_amdgcn_wave_barrier();
+        if (q == ((i - 1) & 3)) { kb = best; kboth = (c1 << 16) | c3; }
     };
 
     // rings (two steps ahead) of census words and, with FUSE, of the other
@@ -505,18 +489,23 @@ __global__ __launch_bounds__(64) void wide_h_kernel(const SgmJob* __restrict__ j
     }
 }
 
+// sums of the fused WTA: 64 * DPL u16 per wave (lines x D), largest at DMAX
+template <int DMAX>
+constexpr int kSumsLen = 64 * wide_dpl(kWideF, DMAX);
+
 template <int DMAX, bool FUSE>
 __global__ __launch_bounds__(64) void wide_v_kernel(const SgmJob* __restrict__ jobs,
                                                     const int* __restrict__ offs, int njobs) {
     __shared__ uint64_t lds[2 * VertLds<DMAX, FUSE ? kWideF : kWideV>::NV];
+    __shared__ __align__(16) uint16_t sums[FUSE ? kSumsLen<DMAX> : 1];
     const int item = blockIdx.x;
     const int j = find_job(offs, njobs, item);
     const SgmJob& J = jobs[j];
     const int group = item - offs[j];
     switch (J.D) {
-        case 64: wide_vertical<64, FUSE>(J, group, threadIdx.x, lds); break;
-        case 128: if constexpr (DMAX >= 128) wide_vertical<128, FUSE>(J, group, threadIdx.x, lds); break;
-        case 256: if constexpr (DMAX >= 256) wide_vertical<256, FUSE>(J, group, threadIdx.x, lds); break;
+        case 64: wide_vertical<64, FUSE>(J, group, threadIdx.x, lds, sums); break;
+        case 128: if constexpr (DMAX >= 128) wide_vertical<128, FUSE>(J, group, threadIdx.x, lds, sums); break;
+        case 256: if constexpr (DMAX >= 256) wide_vertical<256, FUSE>(J, group, threadIdx.x, lds, sums); break;
         default: break;
     }
 }
@@ -555,14 +544,15 @@ __global__ __launch_bounds__(64) void wide_v_persistent(const SgmJob* __restrict
                                                         const uint32_t* __restrict__ items, int n,
                                                         uint32_t* __restrict__ counter) {
     __shared__ uint64_t lds[2 * VertLds<DMAX, FUSE ? kWideF : kWideV>::NV];
+    __shared__ __align__(16) uint16_t sums[FUSE ? kSumsLen<DMAX> : 1];
     uint32_t it;
     while (next_item(items, n, counter, &it)) {
         const SgmJob& J = jobs[it >> 16];
         const int group = (int)(it & 0x3FFFu);
         switch (J.D) {
-            case 64: wide_vertical<64, FUSE>(J, group, threadIdx.x, lds); break;
-            case 128: if constexpr (DMAX >= 128) wide_vertical<128, FUSE>(J, group, threadIdx.x, lds); break;
-            case 256: if constexpr (DMAX >= 256) wide_vertical<256, FUSE>(J, group, threadIdx.x, lds); break;
+            case 64: wide_vertical<64, FUSE>(J, group, threadIdx.x, lds, sums); break;
+            case 128: if constexpr (DMAX >= 128) wide_vertical<128, FUSE>(J, group, threadIdx.x, lds, sums); break;
+            case 256: if constexpr (DMAX >= 256) wide_vertical<256, FUSE>(J, group, threadIdx.x, lds, sums); break;
             default: break;
         }
     }

@@ -56,14 +56,15 @@ class CuSGMOption:
         return o
 
 
-COST_AUTO, COST_STORED = 0, 1
+COST_AUTO, COST_STORED, COST_ON_THE_FLY = 0, 1, 2
 
 
 def set_cost_mode(mode: int) -> None:
     """Where the aggregation takes its matching cost, for handles, batches
-    and pyramids created afterwards: COST_AUTO recomputes it from the census
-    words on the fly where possible, COST_STORED always builds the u8 cost
-    volume.  Results are byte-identical."""
+    and pyramids created afterwards: COST_ON_THE_FLY recomputes it from the
+    census words wherever the views allow, COST_STORED always builds the u8
+    cost volume, COST_AUTO picks on the fly only for plans large enough to
+    fill the GPU.  Results are byte-identical."""
     check(lib().sgm_set_cost_mode(int(mode)))
 
 

@@ -105,14 +105,18 @@ void sgm_roi_from_disparity(const float* disp, int32_t width, int32_t height, sg
 
 /* Where the aggregation takes its matching cost (no reference counterpart;
  * the reference always stores the cost volume, cusgm_cc.cu:118-172):
- *   SGM_COST_AUTO    recompute it from the census words inside the
- *                    aggregation wherever the plan allows (D <= 256, not the
- *                    5x5 right view): no cost volume is built or read
- *   SGM_COST_STORED  always build and read the u8 cost volume
- * Process-wide; read when a handle, batch or pyramid is created.  Both give
+ *   SGM_COST_ON_THE_FLY  recompute it from the census words inside the
+ *                        aggregation wherever the plan allows (D <= 256, not
+ *                        the 5x5 right view): no cost volume is built or read
+ *   SGM_COST_STORED      always build and read the u8 cost volume
+ *   SGM_COST_AUTO        on the fly when the plan also has enough scanlines
+ *                        to fill the device (batches), stored otherwise
+ *                        (single pairs finish sooner that way)
+ * Process-wide; read when a handle, batch or pyramid is created.  All give
  * byte-identical results. */
 #define SGM_COST_AUTO 0
 #define SGM_COST_STORED 1
+#define SGM_COST_ON_THE_FLY 2
 int sgm_set_cost_mode(int32_t mode);
 int32_t sgm_get_cost_mode(void);
 

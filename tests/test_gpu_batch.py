@@ -107,17 +107,38 @@ def _clobber(pairs):
             p["disp_right"].fill_(7.0)
 
 
+@pytest.fixture
+def cost_mode():
+    from fasterstereocuda_amd import stereo
+    before = stereo.get_cost_mode()
+    yield stereo.set_cost_mode
+    stereo.set_cost_mode(before)
+
+
+@pytest.mark.parametrize("mode", [2, 1], ids=["on-the-fly", "stored"])
 @pytest.mark.parametrize("lr", [True, False])
-def test_schedules_give_identical_bytes(cuda, lr):
+def test_schedules_give_identical_bytes(cuda, cost_mode, mode, lr):
     """Grid, job-major and longest-first schedules, with full and tiny
     persistent grids (each wave then drains many queue entries), produce the
-    bytes of the grid schedule, run after run (the queue counters restart)."""
+    bytes of the grid schedule, run after run (the queue counters restart),
+    for both the on-the-fly and the stored-cost kernels; and match the
+    oracle."""
     from fasterstereocuda_amd.batch import SCHED_GRID, SCHED_JOB_MAJOR, SCHED_LONGEST_FIRST
+    cost_mode(mode)
     pairs = _uneven_pairs(cuda, lr)
     b = SgmBatch(pairs, CuSGMOption.benchmark(do_lr_check=lr))
     b.set_schedule(SCHED_GRID)
     b.run()
     ref = _outputs(pairs)
+    k = 0
+    for i, (w, h, d, m) in enumerate(SPECS):
+        sc = scenes.cone_like(i, H=h, W=w, D=d, workload=3)
+        el, er = oracle.match(sc.left, sc.right, m, d, do_lr_check=lr)
+        np.testing.assert_array_equal(ref[k], _bits(el), err_msg=f"pair {i} left vs oracle")
+        k += 1
+        if lr:
+            np.testing.assert_array_equal(ref[k], _bits(er), err_msg=f"pair {i} right vs oracle")
+            k += 1
     for policy in (SCHED_JOB_MAJOR, SCHED_LONGEST_FIRST):
         for waves in (0, 7, 3000):
             print(f"[schedule] lr={lr} policy={policy} waves={waves}", flush=True)
@@ -133,11 +154,13 @@ def test_schedules_give_identical_bytes(cuda, lr):
         b.set_schedule(9)
 
 
+@pytest.mark.parametrize("mode", [2, 1], ids=["on-the-fly", "stored"])
 @pytest.mark.parametrize("policy", [0, 2])
-def test_graph_replays_match_eager(cuda, policy):
+def test_graph_replays_match_eager(cuda, cost_mode, mode, policy):
     """The captured pipeline (kernel nodes only; the queue counters are reset
     by a kernel) replayed several times, with eager torch work and
     synchronisation between replays, gives the eager bytes every time."""
+    cost_mode(mode)
     pairs = _uneven_pairs(cuda, True)
     b = SgmBatch(pairs, CuSGMOption.benchmark())
     b.set_schedule(policy)
