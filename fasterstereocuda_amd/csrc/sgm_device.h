@@ -30,6 +30,14 @@ template <int CTRL>
 __device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
 }
+// The same move where no lane keeps an old value: a lane whose source lies
+// outside its DPP row reads 0 (bound_ctrl).  For patterns whose every source
+// is inside the row (quad_perm, mirrors), or whose out-of-row lanes are
+// overwritten afterwards, it saves the copy of `old` into the destination.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp0(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
 constexpr int kRowShl1 = 0x101;
 constexpr int kRowShr1 = 0x111;
 
@@ -49,6 +57,18 @@ __device__ __forceinline__ int find_job(const int* __restrict__ offs, int n, int
         if (offs[mid] <= item) lo = mid; else hi = mid - 1;
     }
     return lo;
+}
+
+// Workgroups of one launch go round-robin to the 8 XCDs (block b to XCD
+// b mod 8), and each XCD has its own L2.  This bijection on [0, n) gives each
+// XCD one contiguous run of items, so neighbouring items (adjacent column
+// groups that read overlapping census rows) run at the same time under one
+// L2.  On a part with another XCD count it is still a bijection.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_item(int b, int n) {
+    const int q = n / kXcds, r = n % kXcds;
+    const int x = b % kXcds, i = b / kXcds;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
 // Global-address-space views so loads/stores compile to global_* (not flat_*).

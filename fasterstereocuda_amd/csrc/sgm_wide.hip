@@ -49,28 +49,45 @@ constexpr int kRowMirror = 0x140;      // lane i <-> 15 - i within 16 lanes
 // reductions over the LPL lanes of a line (LPL = 4, 8, 16); every lane gets it
 template <int LPL>
 __device__ __forceinline__ uint32_t line_min(uint32_t v) {
-    v = min(v, dpp<kQuadXor1>(0u, v));
-    v = min(v, dpp<kQuadXor2>(0u, v));
-    if constexpr (LPL >= 8) v = min(v, dpp<kRowHalfMirror>(0u, v));
-    if constexpr (LPL >= 16) v = min(v, dpp<kRowMirror>(0u, v));
+    v = min(v, dpp0<kQuadXor1>(v));
+    v = min(v, dpp0<kQuadXor2>(v));
+    if constexpr (LPL >= 8) v = min(v, dpp0<kRowHalfMirror>(v));
+    if constexpr (LPL >= 16) v = min(v, dpp0<kRowMirror>(v));
     return v;
 }
 // value of the previous / next lane of the line; the line's first / last
 // lane gets `edge`
 template <int LPL>
 __device__ __forceinline__ uint32_t from_prev(uint32_t v, uint32_t edge, bool first) {
-    const uint32_t s = dpp<kRowShr1>(edge, v);
-    if constexpr (LPL == 16) return s;
-    else return first ? edge : s;
+    if constexpr (LPL == 16) {
+        return dpp<kRowShr1>(edge, v);
+    } else {
+        const uint32_t s = dpp0<kRowShr1>(v);   // every lane moves; the first then takes `edge`
+        return first ? edge : s;
+    }
 }
 template <int LPL>
 __device__ __forceinline__ uint32_t from_next(uint32_t v, uint32_t edge, bool last) {
-    const uint32_t s = dpp<kRowShl1>(edge, v);
-    if constexpr (LPL == 16) return s;
-    else return last ? edge : s;
+    if constexpr (LPL == 16) {
+        return dpp<kRowShl1>(edge, v);
+    } else {
+        const uint32_t s = dpp0<kRowShl1>(v);
+        return last ? edge : s;
+    }
 }
 
 __device__ __forceinline__ uint32_t popc64(uint64_t a) { return (uint32_t)__popcll(a); }
+// popcount(a) + acc as two accumulating v_bcnt_u32_b32.  Written out, the
+// compiler regroups the sum and packs with a v_lshl_or_b32; this form packs
+// the upper count with a v_lshlrev_b32, which issues at twice the rate.
+__device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
+    uint32_t r;
+    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+    return r;
+}
+__device__ __forceinline__ uint32_t popc_acc(uint64_t a, uint32_t acc) {
+    return bcnt_acc((uint32_t)a, bcnt_acc((uint32_t)(a >> 32), acc));
+}
 
 // One recurrence step of a line, NP packed u16 pairs (2 NP disparities) per lane:
 //   L(d) = C(d) + min(L'(d), B-(d), B+(d), m' + P2) - m'
@@ -124,15 +141,18 @@ __device__ __forceinline__ void line_step(Line<NP>& s, const uint32_t (&c)[NP], 
 }
 
 // the lane's 2 NP bytes (uint8 truncation) in memory order; `sel` reverses
-// the bytes of each word, `rev` the order of the words
+// the bytes of each word, `revmask` (all ones or zero, wave-uniform) the
+// order of the words.  The mask select keeps the word order a bit operation:
+// a conditional on `rev` is turned by the compiler into a per-step indexed
+// select chain.
 template <int NP>
-__device__ __forceinline__ void pack_lane(const Line<NP>& s, uint32_t sel, bool rev,
+__device__ __forceinline__ void pack_lane(const Line<NP>& s, uint32_t sel, uint32_t revmask,
                                           uint32_t (&o)[NP / 2]) {
     uint32_t w[NP / 2];
 #pragma unroll
     for (int k = 0; k < NP / 2; ++k) w[k] = __builtin_amdgcn_perm(s.L[2 * k + 1], s.L[2 * k], sel);
 #pragma unroll
-    for (int k = 0; k < NP / 2; ++k) o[k] = rev ? w[NP / 2 - 1 - k] : w[k];
+    for (int k = 0; k < NP / 2; ++k) o[k] = (w[NP / 2 - 1 - k] & revmask) | (w[k] & ~revmask);
 }
 
 constexpr uint32_t kSelFwd = 0x06040200u;   // bytes d, d+1, d+2, d+3
@@ -147,10 +167,10 @@ template <int LPL>
 __device__ __forceinline__ uint32_t line_bcast(uint32_t v, int src, int lane0) {
     if constexpr (LPL == 4) {
         switch (src) {   // quad_perm [src, src, src, src]
-            case 0: return dpp<0x00>(0u, v);
-            case 1: return dpp<0x55>(0u, v);
-            case 2: return dpp<0xAA>(0u, v);
-            default: return dpp<0xFF>(0u, v);
+            case 0: return dpp0<0x00>(v);
+            case 1: return dpp0<0x55>(v);
+            case 2: return dpp0<0xAA>(v);
+            default: return dpp0<0xFF>(v);
         }
     } else {
         return (uint32_t)__builtin_amdgcn_ds_bpermute((lane0 + src) << 2, (int)v);
@@ -168,7 +188,7 @@ template <int NP, class F>
 __device__ __forceinline__ void lane_costs(uint64_t own, F&& partner, uint32_t (&c)[NP]) {
 #pragma unroll
     for (int k = 0; k < NP; ++k)
-        c[k] = popc64(own ^ partner(2 * k)) | (popc64(own ^ partner(2 * k + 1)) << 16);
+        c[k] = popc_acc(own ^ partner(2 * k), popc64(own ^ partner(2 * k + 1)) << 16);
 }
 
 // ------------------------------------------------------------ horizontal ---
@@ -197,6 +217,7 @@ __device__ __forceinline__ void wide_horizontal(const SgmJob& J, int path, int g
     const uint64_t* cm = J.census_other + (int64_t)y * W;
     const uint32_t P1v = (uint32_t)J.p1 * 0x10001u, P2v = (uint32_t)J.p2 * 0x10001u;
     const uint32_t sel = rev ? kSelRev : kSelFwd;
+    const uint32_t revmask = rev ? ~0u : 0u;
     const int chunk = rev ? LPL - 1 - q : q;
     uint8_t* const out_base = (path == kLR) ? J.path[kLR] : J.path[kRL];
     uint8_t* const dst0 = out_base + ((int64_t)y * W + x0) * D + chunk * DPL;
@@ -212,7 +233,7 @@ __device__ __forceinline__ void wide_horizontal(const SgmJob& J, int path, int g
         lane_costs<NP>(g64(cb + x0), [&](int j) { return win[j]; }, c);
         line_init<LPL>(s, c);
         uint32_t o[NP / 2];
-        pack_lane(s, sel, rev, o);
+        pack_lane(s, sel, revmask, o);
         if (active) store_words<NP / 2>(dst0, o);
     }
 
@@ -254,7 +275,7 @@ __device__ __forceinline__ void wide_horizontal(const SgmJob& J, int path, int g
             lane_costs<NP>(own, [&](int j) { return win[(e + j) & (DPL - 1)]; }, c);
             line_step<LPL>(s, c, P1v, P2v, first, last);
             uint32_t o[NP / 2];
-            pack_lane(s, sel, rev, o);
+            pack_lane(s, sel, revmask, o);
             if (active && i < n) store_words<NP / 2>(dst0 + (int64_t)i * dstep, o);
         }
 #pragma unroll
@@ -432,7 +453,7 @@ __device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int la
         line_init<LPL>(s, c);
         if constexpr (!FUSE) {
             uint32_t o[NW];
-            pack_lane(s, kSelFwd, false, o);
+            pack_lane(s, kSelFwd, 0u, o);
             if (active) store_words<NW>(out_base + off0, o);
         } else {
             // row H - 1 is a border row of the WTA
@@ -452,7 +473,7 @@ __device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int la
             line_step<LPL>(s, c, P1v, P2v, first, last);
             if constexpr (!FUSE) {
                 uint32_t o[NW];
-                pack_lane(s, kSelFwd, false, o);
+                pack_lane(s, kSelFwd, 0u, o);
                 if (active && i < H) store_words<NW>(out_base + off0 + (int64_t)i * ostep, o);
             } else {
                 // consume the slot, then refill it for step i + PF
@@ -498,7 +519,7 @@ __global__ __launch_bounds__(64) void wide_v_kernel(const SgmJob* __restrict__ j
                                                     const int* __restrict__ offs, int njobs) {
     __shared__ uint64_t lds[2 * VertLds<DMAX, FUSE ? kWideF : kWideV>::NV];
     __shared__ __align__(16) uint16_t sums[FUSE ? kSumsLen<DMAX> : 1];
-    const int item = blockIdx.x;
+    const int item = xcd_item(blockIdx.x, gridDim.x);
     const int j = find_job(offs, njobs, item);
     const SgmJob& J = jobs[j];
     const int group = item - offs[j];
