@@ -74,7 +74,12 @@ enum WideKind { kWideH = 0, kWideV = 1, kWideF = 2 };   // LR + RL, UD, DU + WTA
 #endif
 __host__ __device__ constexpr int wide_dpl(int kind, int D) {
     const int want = kind == kWideH ? FSGM_DPL_H : kind == kWideV ? FSGM_DPL_V : FSGM_DPL_F;
-    return want * 16 >= D ? want : D / 16;   // D / DPL <= 16
+    int dpl = want * 16 >= D ? want : D / 16;   // D / DPL <= 16
+    // horizontal: a block of DPL steps spreads its census loads over the
+    // line's D / DPL lanes, so DPL >= D / DPL
+    if (kind == kWideH)
+        while (dpl * dpl < D) dpl *= 2;
+    return dpl;
 }
 __host__ __device__ constexpr int wide_lines(int kind, int D) { return 64 * wide_dpl(kind, D) / D; }
 inline int wide_h_items(const SgmJob& j) {
@@ -87,7 +92,12 @@ inline int wide_v_items(const SgmJob& j, int kind) {
 }
 // Whether a job's cost can come from the census on the fly: D <= 256, and not
 // the 5x5 right view, whose skipped-pixel rule needs the stored cost.
-inline bool wide_ok(const SgmJob& j) { return j.D <= 256 && !(j.is_right && j.cs_type == 0); }
+// The horizontal kernel addresses a wave's rows through a raw buffer with
+// 31-bit offsets: lines-per-wave * W * D bytes must stay below 2^31.
+inline bool wide_ok(const SgmJob& j) {
+    return j.D <= 256 && !(j.is_right && j.cs_type == 0) &&
+           (int64_t)wide_lines(kWideH, j.D) * j.W * j.D < ((int64_t)1 << 31);
+}
 
 // Device-side launch plan: jobs, per-stage prefix sums and totals.
 struct StagePlan {
