@@ -92,12 +92,7 @@ inline int wide_v_items(const SgmJob& j, int kind) {
 }
 // Whether a job's cost can come from the census on the fly: D <= 256, and not
 // the 5x5 right view, whose skipped-pixel rule needs the stored cost.
-// The horizontal kernel addresses a wave's rows through a raw buffer with
-// 31-bit offsets: lines-per-wave * W * D bytes must stay below 2^31.
-inline bool wide_ok(const SgmJob& j) {
-    return j.D <= 256 && !(j.is_right && j.cs_type == 0) &&
-           (int64_t)wide_lines(kWideH, j.D) * j.W * j.D < ((int64_t)1 << 31);
-}
+inline bool wide_ok(const SgmJob& j) { return j.D <= 256 && !(j.is_right && j.cs_type == 0); }
 
 // Device-side launch plan: jobs, per-stage prefix sums and totals.
 struct StagePlan {
