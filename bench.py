@@ -9,11 +9,15 @@ outputs resident in HBM.  The headline value is the batched path
 
 Prints ONE JSON line:
   value          W*H*D/s summed over the batch (batched path)
-  roofline       aggregation + WTA (two launches: three directions, then
-                 bottom->top fused with the WTA): algorithmic 8 B/cell/view
-                 (4 B cost reads + 2 B summed output + 2 B read by the WTA) /
-                 their average duration from HIP events recorded around the
-                 launches during the timed steps
+  roofline       aggregation + WTA (the launches between the census and the
+                 post stage: with the cost computed on the fly, left->right +
+                 right->left, top->bottom, then bottom->top fused with the
+                 WTA): algorithmic 8 B/cell/view (4 B cost reads + 2 B summed
+                 output + 2 B read by the WTA) / their average duration from
+                 HIP events recorded around them during the timed steps;
+                 `traffic` is the measured HBM bytes of the same launches from
+                 the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+                 this workload
   cpu_baseline   the OpenMP oracle on a bounded sample of the same pairs
 """
 from __future__ import annotations
@@ -30,8 +34,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per cell and view of aggregation + WTA: 4 direction reads
 # of the u8 cost + one u16 4-direction sum write (6 B), and the WTA's read of
-# that sum (2 B).  The engine runs them as two launches: three directions,
-# then bottom->top fused with the WTA.
+# that sum (2 B).  The engine runs them as the horizontal and top->bottom
+# launches, then bottom->top fused with the WTA.
 AGG_BYTES_PER_CELL = 8.0
 
 
