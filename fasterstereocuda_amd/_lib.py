@@ -44,6 +44,13 @@ class SgmOption(ctypes.Structure):
     ]
 
 
+class SgmCamera(ctypes.Structure):
+    """Field-for-field mirror of sgm_camera (CamParam_T, sgm_types.h:69-84)."""
+    _fields_ = [("x0_left", ctypes.c_double), ("y0_left", ctypes.c_double),
+                ("x0_right", ctypes.c_double), ("y0_right", ctypes.c_double),
+                ("focus", ctypes.c_double), ("baseline", ctypes.c_double)]
+
+
 class SgmRoi(ctypes.Structure):
     _fields_ = [("x", ctypes.c_int32), ("y", ctypes.c_int32),
                 ("w", ctypes.c_int32), ("h", ctypes.c_int32)]
@@ -76,6 +83,10 @@ SIGNATURES = {
     "sgm_init": (ctypes.c_int, [P, I32, I32, I32, I32, ctypes.POINTER(SgmOption), I32]),
     "sgm_match": (ctypes.c_int, [P, P, P, P, P, P, P, P]),
     "sgm_match_device": (ctypes.c_int, [P, P, P, P, P, P]),
+    "sgm_depth_device": (ctypes.c_int, [P, I32, I32, ctypes.POINTER(SgmCamera), P, P]),
+    "sgm_init2": (ctypes.c_int, [P, I32, I32, I32, I32, ctypes.POINTER(SgmOption),
+                                 ctypes.POINTER(SgmCamera), I32]),
+    "sgm_match2": (ctypes.c_int, [P, P, P, P, P, P, P]),
     "sgm_set_min_disparity": (ctypes.c_int, [P, I32]),
     "sgm_get_disparity_range": (I32, [P]),
     "sgm_release": (ctypes.c_int, [P]),
@@ -108,6 +119,10 @@ SIGNATURES = {
     "sgm_pyramid_create": (P, [I32, I32, I32, I32, I32, I32, ctypes.POINTER(SgmOption)]),
     "sgm_pyramid_match": (ctypes.c_int, [P, P, P, P]),
     "sgm_pyramid_match_device": (ctypes.c_int, [P, P, P, P, P]),
+    "sgm_pyramid_create2": (P, [I32, I32, F32, F32, I32, I32, ctypes.POINTER(SgmOption),
+                                ctypes.POINTER(SgmCamera)]),
+    "sgm_pyramid_match2": (ctypes.c_int, [P, P, P, P]),
+    "sgm_pyramid_match2_device": (ctypes.c_int, [P, P, P, P, P]),
     "sgm_pyramid_coarsest_range": (I32, [P, ctypes.POINTER(I32)]),
     "sgm_pyramid_device_bytes": (ctypes.c_size_t, [P]),
     "sgm_pyramid_destroy": (None, [P]),

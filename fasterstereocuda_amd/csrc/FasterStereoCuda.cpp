@@ -1,6 +1,7 @@
 // FasterStereoCuda drop-in class: a pimpl over the pyramid entry points of
 // the C ABI (sgm_capi.h).  Follows the reference's FasterStereoCuda.cpp:
-// 300-560 (class shell) and 169-199 (per-level options): the level-dependent
+// 203-406 (Init, Init2), 408-601 (Match, Match2), 693-766 (class shell) and
+// 169-199 (per-level options): the level-dependent
 // parts (uniqueness 1.0 above level 0, LR check only at the coarsest level,
 // dilation of the coarsest level) are applied inside sgm_pyramid_*.
 #include "FasterStereoCuda.h"
@@ -21,7 +22,7 @@ inline Impl* I(void* p) { return static_cast<Impl*>(p); }
 
 constexpr int kLevelRange = 32;   // window of the finer levels (reference LEVEL_RANGE)
 
-sgm_option level_options(const FasterStereoCuda::ste_opt1_t& o) {
+sgm_option level_options(bool do_lr_check, bool do_rm_peaks, bool do_smooth) {
     sgm_option c{};
     c.p1 = 8;
     c.p2_init = 32;
@@ -29,13 +30,13 @@ sgm_option level_options(const FasterStereoCuda::ste_opt1_t& o) {
     c.cs_type = 0;                       // 5x5 census
     c.num_paths = 4;
     c.do_median_filter = 1;
-    c.do_lr_check = o.do_lr_check ? 1 : 0;
+    c.do_lr_check = do_lr_check ? 1 : 0;
     c.lr_threshold = 1.0f;
     c.unique_threshold = 0.95f;
     c.using_constant_p2 = 1;
-    c.do_remove_peaks = o.do_rm_peaks ? 1 : 0;
+    c.do_remove_peaks = do_rm_peaks ? 1 : 0;
     c.peaks_ratio_threshold = 0.0005f;
-    c.post_filter_type = o.do_smooth ? 1 : 0;
+    c.post_filter_type = do_smooth ? 1 : 0;
     c.morphology_type = 0;
     return c;
 }
@@ -63,7 +64,7 @@ bool FasterStereoCuda::Init(const ste_opt1_t& option) const {
         m->err = "image size error";
         return false;
     }
-    const sgm_option c = level_options(option);
+    const sgm_option c = level_options(option.do_lr_check, option.do_rm_peaks, option.do_smooth);
     m->pyr = sgm_pyramid_create(option.width, option.height, option.min_disp, option.max_disp,
                                 std::max(1, option.num_layers), kLevelRange, &c);
     if (!m->pyr) {
@@ -74,9 +75,24 @@ bool FasterStereoCuda::Init(const ste_opt1_t& option) const {
     return true;
 }
 
-bool FasterStereoCuda::Init2(const ste_opt2_t&) const {
-    I(impl_)->err = "depth output (Init2 / Match2) is not built";
-    return false;
+bool FasterStereoCuda::Init2(const ste_opt2_t& option) const {
+    Impl* m = I(impl_);
+    Release();
+    if (option.width <= 0 || option.height <= 0) {
+        m->err = "image size error";
+        return false;
+    }
+    const sgm_option c = level_options(option.do_lr_check, option.do_rm_peaks, option.do_smooth);
+    const sgm_camera cam{option.epi.x0_left,  option.epi.y0_left, option.epi.x0_right,
+                         option.epi.y0_right, option.epi.focus,   option.epi.baseline};
+    m->pyr = sgm_pyramid_create2(option.width, option.height, option.min_depth, option.max_depth,
+                                 std::max(1, option.num_layers), kLevelRange, &c, &cam);
+    if (!m->pyr) {
+        m->err = sgm_last_error(nullptr);
+        return false;
+    }
+    m->err.clear();
+    return true;
 }
 
 void FasterStereoCuda::Release() const {
@@ -105,9 +121,22 @@ bool FasterStereoCuda::Match(const unsigned char* bytes_left, const unsigned cha
     return true;
 }
 
-bool FasterStereoCuda::Match2(const unsigned char*, const unsigned char*, float*) const {
-    I(impl_)->err = "depth output (Init2 / Match2) is not built";
-    return false;
+bool FasterStereoCuda::Match2(const unsigned char* bytes_left, const unsigned char* bytes_right,
+                              float* depth_data) const {
+    Impl* m = I(impl_);
+    if (!m->pyr) {
+        m->err = "not initialised";
+        return false;
+    }
+    if (!bytes_left || !bytes_right || !depth_data) {
+        m->err = "NULL image or output pointer";
+        return false;
+    }
+    if (sgm_pyramid_match2(m->pyr, bytes_left, bytes_right, depth_data) != SGM_OK) {
+        m->err = sgm_last_error(nullptr);
+        return false;
+    }
+    return true;
 }
 
 const char* FasterStereoCuda::LastError() const { return I(impl_)->err.c_str(); }

@@ -59,12 +59,24 @@ bool StereoCuda::Match(uint8* img_left, uint8* img_right, float32* disp_left, fl
                      reinterpret_cast<const sgm_roi*>(ste_roi_right)) == SGM_OK;
 }
 
-bool StereoCuda::Init2(sint32, sint32, sint32, sint32, CuSGMOption, CamParam_T, bool) const {
-    return false;
+bool StereoCuda::Init2(sint32 width, sint32 height, sint32 min_disparity, sint32 disp_range,
+                       CuSGMOption option, CamParam_T cam_param, bool print_log) const {
+    if (!impl_) return false;
+    const sgm_option c = to_c(option);
+    const sgm_camera cam{cam_param.x0_left,  cam_param.y0_left, cam_param.x0_right,
+                         cam_param.y0_right, cam_param.focus,   cam_param.baseline};
+    const int rc = sgm_init2(H(impl_), width, height, min_disparity, disp_range, &c, &cam, print_log);
+    if (rc != SGM_OK && print_log) std::printf("StereoCuda::Init2: %s\n", sgm_last_error(H(impl_)));
+    return rc == SGM_OK;
 }
 
-bool StereoCuda::Match2(uint8*, uint8*, float32*, sint16*, StereoROI_T*, StereoROI_T*) const {
-    return false;
+bool StereoCuda::Match2(uint8* img_left, uint8* img_right, float32* depth_left,
+                        sint16* init_disp_left, StereoROI_T* ste_roi_left,
+                        StereoROI_T* ste_roi_right) const {
+    if (!impl_) return false;
+    return sgm_match2(H(impl_), img_left, img_right, depth_left, init_disp_left,
+                      reinterpret_cast<const sgm_roi*>(ste_roi_left),
+                      reinterpret_cast<const sgm_roi*>(ste_roi_right)) == SGM_OK;
 }
 
 void StereoCuda::SetMinDisparity(const sint32& min_disparty) {

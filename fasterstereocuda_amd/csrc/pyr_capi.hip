@@ -13,6 +13,7 @@
 // per level; here every step is a kernel on one stream.
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -54,6 +55,8 @@ struct sgm_pyramid {
     void* top_block = nullptr;
     std::vector<StagePlan> lvl_plan;
     std::vector<void*> lvl_block;
+    bool has_cam = false;         // set by sgm_pyramid_create2 (level-0 camera)
+    sgm_camera cam{};
 };
 
 namespace {
@@ -79,34 +82,53 @@ SgmJob census_job(const uint8_t* img, uint64_t* cs, const uint64_t* other, int w
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-sgm_pyramid* sgm_pyramid_create(int32_t width, int32_t height, int32_t min_disp, int32_t max_disp,
-                                int32_t num_layers, int32_t level_range, const sgm_option* opt) {
-    if (width <= 0 || height <= 0 || max_disp <= min_disp) {
-        set_global_error(SGM_E_INVALID_ARG, "pyramid: bad size or disparity interval");
-        return nullptr;
+// Checks shared by both constructors; returns false with the error set.
+bool pyramid_args_ok(int32_t width, int32_t height, int32_t num_layers, int32_t level_range,
+                     const sgm_option* opt) {
+    if (width <= 0 || height <= 0) {
+        set_global_error(SGM_E_INVALID_ARG, "pyramid: bad size");
+        return false;
     }
     if (level_range != 16 && level_range != 32) {
         set_global_error(SGM_E_INVALID_ARG, "pyramid: level_range must be 16 or 32");
-        return nullptr;
+        return false;
     }
     const int layers = std::max(1, num_layers);
     if ((height >> (layers - 1)) < 8 || (width >> (layers - 1)) < 8) {
         set_global_error(SGM_E_INVALID_ARG, "pyramid: too many layers for the image size");
-        return nullptr;
+        return false;
     }
     if (!opt || opt->cs_type < 0 || opt->cs_type > 1 || opt->p1 < 0 || opt->p2_init < 0 ||
         (int64_t)opt->p1 + opt->p2_init > 65000) {
         set_global_error(SGM_E_INVALID_ARG, "pyramid: bad options");
-        return nullptr;
+        return false;
     }
     if (opt->do_remove_peaks || opt->post_filter_type != 0 || opt->morphology_type != 0 ||
         !opt->using_constant_p2) {
         set_global_error(SGM_E_UNSUPPORTED,
                          "pyramid: remove-peaks, post filters, morphology and adaptive P2 are not built");
-        return nullptr;
+        return false;
     }
+    return true;
+}
+
+// Coarsest-level interval, shared by both constructors
+// (FasterStereoCuda.cpp:266-281 and :373-391): the smallest 64 * 2^k covering
+// span, halved once if above twice the (full) width; a minimum at or past the
+// width moves to width - range, not below -width.
+void coarsest_interval(double span, int width, int* dmin, int* range) {
+    int r = 64;
+    while (r < span && r < (1 << 20)) r <<= 1;   // beyond 512 the caller rejects it
+    if (r > 2 * width) r >>= 1;
+    *range = r;
+    if (*dmin >= width) *dmin = std::max(width - r, -width);
+}
+
+sgm_pyramid* build_pyramid(int32_t width, int32_t height, int32_t num_layers, int32_t level_range,
+                           const sgm_option* opt, int dmin, int range) {
+    const int layers = std::max(1, num_layers);
     sgm_pyramid* p = new (std::nothrow) sgm_pyramid();
     if (!p) return nullptr;
     p->opt = *opt;
@@ -118,14 +140,6 @@ sgm_pyramid* sgm_pyramid_create(int32_t width, int32_t height, int32_t min_disp,
     }
     const int top = layers - 1;
     for (int i = 0; i < layers; ++i) { p->lw.push_back(width >> i); p->lh.push_back(height >> i); }
-    // coarsest range: FasterStereoCuda.cpp:265-281 (compares with the full width)
-    const int scale = 1 << top;
-    int dmin = min_disp / scale;
-    const int dmax = max_disp / scale;
-    int range = 64;
-    while (range < dmax - dmin) range <<= 1;
-    if (range > 2 * width) range >>= 1;
-    if (dmin >= width) dmin = std::max(width - range, -width);
     if (range > 512 || std::max(0, dmin) >= p->lw[top]) {
         set_global_error(SGM_E_INVALID_ARG, "pyramid: coarsest disparity range not supported");
         delete p;
@@ -221,6 +235,56 @@ sgm_pyramid* sgm_pyramid_create(int32_t width, int32_t height, int32_t min_disp,
     return p;
 }
 
+}  // namespace
+
+extern "C" {
+
+sgm_pyramid* sgm_pyramid_create(int32_t width, int32_t height, int32_t min_disp, int32_t max_disp,
+                                int32_t num_layers, int32_t level_range, const sgm_option* opt) {
+    if (max_disp <= min_disp) {
+        set_global_error(SGM_E_INVALID_ARG, "pyramid: bad disparity interval");
+        return nullptr;
+    }
+    if (!pyramid_args_ok(width, height, num_layers, level_range, opt)) return nullptr;
+    // FasterStereoCuda.cpp:265-281: integer division by the coarsest scale
+    const int scale = 1 << (std::max(1, num_layers) - 1);
+    int dmin = min_disp / scale, range = 0;
+    coarsest_interval((double)(max_disp / scale - dmin), width, &dmin, &range);
+    return build_pyramid(width, height, num_layers, level_range, opt, dmin, range);
+}
+
+sgm_pyramid* sgm_pyramid_create2(int32_t width, int32_t height, float min_depth, float max_depth,
+                                 int32_t num_layers, int32_t level_range, const sgm_option* opt,
+                                 const sgm_camera* cam) {
+    if (!cam || !(min_depth > 0.0f) || !(max_depth > min_depth) || !std::isfinite(max_depth) ||
+        !std::isfinite(cam->focus) || !std::isfinite(cam->baseline) ||
+        !std::isfinite(cam->x0_left) || !std::isfinite(cam->x0_right)) {
+        set_global_error(SGM_E_INVALID_ARG, "pyramid: bad camera or depth interval");
+        return nullptr;
+    }
+    if (!pyramid_args_ok(width, height, num_layers, level_range, opt)) return nullptr;
+    // FasterStereoCuda.cpp:373-391 with the camera scaled by 1/2^top
+    // (CamParam_T::Scale, sgm_types.h:75-82), all in float64
+    const double s = (double)(1 << (std::max(1, num_layers) - 1));
+    const double x0l = cam->x0_left / s, x0r = cam->x0_right / s, f = cam->focus / s;
+    const double dmin_f = cam->baseline * f / max_depth - x0r + x0l;
+    const double dmax_f = cam->baseline * f / min_depth - x0r + x0l;
+    if (!std::isfinite(dmin_f) || !std::isfinite(dmax_f) || dmin_f < -1e9 || dmax_f > 1e9) {
+        set_global_error(SGM_E_INVALID_ARG, "pyramid: depth interval maps to no disparity interval");
+        return nullptr;
+    }
+    // the reference tests the float64 minimum against the width and floors
+    // it afterwards; floor(x) >= width exactly when x >= width
+    int dmin = (int)std::floor(dmin_f), range = 0;
+    coarsest_interval(dmax_f - dmin_f, width, &dmin, &range);
+    sgm_pyramid* p = build_pyramid(width, height, num_layers, level_range, opt, dmin, range);
+    if (p) {
+        p->cam = *cam;   // level 0 uses the unscaled camera (CamParam_T::Scale(1))
+        p->has_cam = true;
+    }
+    return p;
+}
+
 }  // extern "C"
 
 namespace {
@@ -276,6 +340,41 @@ int sgm_pyramid_match_device(sgm_pyramid* p, const uint8_t* d_left, const uint8_
     PYR_TRY(hipMemcpyAsync(p->ir[0], d_right, npix0, hipMemcpyDeviceToDevice, s));
     if (int rc = run_pyramid(p, s)) return rc;
     PYR_TRY(hipMemcpyAsync(d_disp, p->out[0], npix0 * 4, hipMemcpyDeviceToDevice, s));
+    return SGM_OK;
+}
+
+int sgm_pyramid_match2_device(sgm_pyramid* p, const uint8_t* d_left, const uint8_t* d_right,
+                              float* d_depth, void* stream) {
+    if (!p || !d_left || !d_right || !d_depth)
+        return set_global_error(SGM_E_INVALID_ARG, "pyramid: NULL argument");
+    if (!p->has_cam)
+        return set_global_error(SGM_E_INVALID_ARG, "pyramid: Match2 needs a pyramid from Init2");
+    PYR_TRY(hipSetDevice(p->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t npix0 = (size_t)p->W * p->H;
+    PYR_TRY(hipMemcpyAsync(p->il[0], d_left, npix0, hipMemcpyDeviceToDevice, s));
+    PYR_TRY(hipMemcpyAsync(p->ir[0], d_right, npix0, hipMemcpyDeviceToDevice, s));
+    if (int rc = run_pyramid(p, s)) return rc;
+    PYR_TRY(launch_depth(p->out[0], d_depth, npix0, p->cam.baseline * p->cam.focus,
+                         p->cam.x0_right - p->cam.x0_left, s));
+    return SGM_OK;
+}
+
+int sgm_pyramid_match2(sgm_pyramid* p, const uint8_t* left, const uint8_t* right, float* depth) {
+    if (!p || !left || !right || !depth)
+        return set_global_error(SGM_E_INVALID_ARG, "pyramid: NULL argument");
+    if (!p->has_cam)
+        return set_global_error(SGM_E_INVALID_ARG, "pyramid: Match2 needs a pyramid from Init2");
+    PYR_TRY(hipSetDevice(p->device));
+    const size_t npix0 = (size_t)p->W * p->H;
+    PYR_TRY(hipMemcpyAsync(p->il[0], left, npix0, hipMemcpyHostToDevice, p->stream));
+    PYR_TRY(hipMemcpyAsync(p->ir[0], right, npix0, hipMemcpyHostToDevice, p->stream));
+    if (int rc = run_pyramid(p, p->stream)) return rc;
+    // level 0's raw map is free once its post stage has read it
+    PYR_TRY(launch_depth(p->out[0], p->raw[0], npix0, p->cam.baseline * p->cam.focus,
+                         p->cam.x0_right - p->cam.x0_left, p->stream));
+    PYR_TRY(hipMemcpyAsync(depth, p->raw[0], npix0 * 4, hipMemcpyDeviceToHost, p->stream));
+    PYR_TRY(hipStreamSynchronize(p->stream));
     return SGM_OK;
 }
 

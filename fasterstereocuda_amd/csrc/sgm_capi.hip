@@ -261,6 +261,8 @@ struct sgm_handle {
     void* plan_block = nullptr;
     ViewBuffers view[2];
     StagePlan plan;
+    bool has_cam = false;         // set by sgm_init2, used by sgm_match2
+    sgm_camera cam{};
     std::string err;
 };
 
@@ -436,6 +438,50 @@ int sgm_match(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
     return SGM_OK;
 }
 
+int sgm_depth_device(const float* d_disp, int32_t width, int32_t height, const sgm_camera* cam,
+                     float* d_depth, void* stream) {
+    if (!d_disp || !d_depth || !cam || width <= 0 || height <= 0)
+        return fail(nullptr, SGM_E_INVALID_ARG, "depth: NULL pointer or bad size");
+    HIP_TRY(nullptr, launch_depth(d_disp, d_depth, (size_t)width * height,
+                                  cam->baseline * cam->focus, cam->x0_right - cam->x0_left,
+                                  static_cast<hipStream_t>(stream)));
+    return SGM_OK;
+}
+
+int sgm_init2(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity,
+              int32_t disp_range, const sgm_option* opt, const sgm_camera* cam, int32_t print_log) {
+    if (!h) return fail(nullptr, SGM_E_INVALID_ARG, "handle is NULL");
+    if (!cam) return fail(&h->err, SGM_E_INVALID_ARG, "camera is NULL");
+    if (int rc = sgm_init(h, width, height, min_disparity, disp_range, opt, print_log)) return rc;
+    h->cam = *cam;
+    h->has_cam = true;
+    return SGM_OK;
+}
+
+int sgm_match2(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right, float* depth_left,
+               const int16_t* init_disp_left, const sgm_roi* roi_left, const sgm_roi* roi_right) {
+    if (int rc = check_ready(h)) return rc;
+    if (!h->has_cam) return fail(&h->err, SGM_E_INVALID_ARG, "Match2 needs a handle set up by Init2");
+    if (!img_left || !img_right || !depth_left)
+        return fail(&h->err, SGM_E_INVALID_ARG, "image or output pointer is NULL");
+    if (init_disp_left)
+        return fail(&h->err, SGM_E_UNSUPPORTED,
+                    "initial disparities are used through the pyramid entry points");
+    if (roi_left || roi_right) return fail(&h->err, SGM_E_UNSUPPORTED, "ROIs are not built");
+    const size_t npix = (size_t)h->W * h->H;
+    hipStream_t s = h->stream;
+    HIP_TRY(&h->err, hipMemcpyAsync(h->view[0].img, img_left, npix, hipMemcpyHostToDevice, s));
+    HIP_TRY(&h->err, hipMemcpyAsync(h->view[1].img, img_right, npix, hipMemcpyHostToDevice, s));
+    if (int rc = run_pipeline(h, false, s)) return rc;
+    // the raw left map is free once the post stage has read it
+    float* depth = h->view[0].raw;
+    HIP_TRY(&h->err, launch_depth(h->view[0].out, depth, npix, h->cam.baseline * h->cam.focus,
+                                  h->cam.x0_right - h->cam.x0_left, s));
+    HIP_TRY(&h->err, hipMemcpyAsync(depth_left, depth, npix * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(&h->err, hipStreamSynchronize(s));
+    return SGM_OK;
+}
+
 int sgm_set_min_disparity(sgm_handle* h, int32_t min_disparity) {
     if (int rc = check_ready(h)) return rc;
     if (std::max(0, min_disparity) >= h->W)
@@ -459,6 +505,7 @@ int sgm_release(sgm_handle* h) {
     for (auto& v : h->view) v = ViewBuffers{};
     h->plan = StagePlan{};
     h->inited = false;
+    h->has_cam = false;
     return SGM_OK;
 }
 

@@ -154,6 +154,9 @@ hipError_t launch_wta(const StagePlan& p, hipStream_t s);
 hipError_t launch_post(const StagePlan& p, hipStream_t s);
 hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s);
 hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t s);
+// depth = (float)(f1 / ((double)disp + f2)), invalid kept (n pixels)
+hipError_t launch_depth(const float* disp, float* depth, size_t n, double f1, double f2,
+                        hipStream_t s);
 
 // sgm_capi.hip: the matching sequence shared by the plain, batched and
 // pyramid paths (marks: 6 events or null)

@@ -636,7 +636,27 @@ __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ dst, size
         dst[i] = v;
 }
 
+// Depth from disparity (cusgm_util.cu:7-33): f1 = baseline * focus and
+// f2 = x0_right - x0_left, both float64; the division is IEEE float64 and the
+// result is rounded once to float32.  Invalid pixels stay invalid.
+__global__ __launch_bounds__(256) void depth_kernel(const float* __restrict__ disp,
+                                                    float* __restrict__ depth, size_t n,
+                                                    double f1, double f2) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const float d = disp[i];
+        depth[i] = d == kInvalidDisp ? kInvalidDisp : (float)(f1 / ((double)d + f2));
+    }
+}
+
 }  // namespace
+
+hipError_t launch_depth(const float* disp, float* depth, size_t n, double f1, double f2,
+                        hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
+    hipLaunchKernelGGL(depth_kernel, dim3(blocks), dim3(256), 0, s, disp, depth, n, f1, f2);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t s) {
     if (n == 0) return hipSuccess;

@@ -4,10 +4,14 @@
 //   stereo_demo LEFT.u8 RIGHT.u8 W H MIN_D RANGE OUT_LEFT.f32 [OUT_RIGHT.f32]
 //               [--cs 0|1] [--p1 N] [--p2 N] [--uniq F] [--lr 0|1] [--lrthr F]
 //               [--median 0|1] [--repeat N] [--layers N]
+//               [--cam X0L,X0R,FOCUS,BASELINE] [--zrange MIN,MAX]
 //
 // Without --layers: StereoCuda (single level, the options above).  With
 // --layers N: FasterStereoCuda (N-level pyramid over [MIN_D, MIN_D + RANGE),
 // its own fixed per-level options; only --lr and --repeat apply).
+// With --cam the output is depth instead of disparity, through Init2 /
+// Match2: StereoCuda keeps MIN_D and RANGE; FasterStereoCuda takes its
+// interval from --zrange (the depth interval) and ignores MIN_D and RANGE.
 // Inputs are tightly packed uint8 W x H; outputs tightly packed float32.
 // Prints the average Match time over --repeat calls after the first.
 #include <chrono>
@@ -60,6 +64,9 @@ int main(int argc, char** argv) {
     opt.do_median_filter = true;
     opt.using_constant_p2 = true;
     int repeat = 1, layers = 0;
+    bool depth = false;
+    double cam[4] = {0, 0, 0, 0};      // x0_left, x0_right, focus, baseline
+    float zrange[2] = {0.0f, 0.0f};
     for (; a + 1 < argc; a += 2) {
         const std::string k = argv[a];
         const char* v = argv[a + 1];
@@ -72,6 +79,15 @@ int main(int argc, char** argv) {
         else if (k == "--median") opt.do_median_filter = std::atoi(v) != 0;
         else if (k == "--repeat") repeat = std::atoi(v);
         else if (k == "--layers") layers = std::atoi(v);
+        else if (k == "--cam") {
+            depth = std::sscanf(v, "%lf,%lf,%lf,%lf", &cam[0], &cam[1], &cam[2], &cam[3]) == 4;
+            if (!depth) { std::fprintf(stderr, "--cam needs X0L,X0R,FOCUS,BASELINE\n"); return 2; }
+        } else if (k == "--zrange") {
+            if (std::sscanf(v, "%f,%f", &zrange[0], &zrange[1]) != 2) {
+                std::fprintf(stderr, "--zrange needs MIN,MAX\n");
+                return 2;
+            }
+        }
         else { std::fprintf(stderr, "unknown option %s\n", k.c_str()); return 2; }
     }
 
@@ -85,18 +101,31 @@ int main(int argc, char** argv) {
 
     if (layers > 0) {
         FasterStereoCuda fs;
-        FasterStereoCuda::ste_opt1_t o;
-        o.width = W; o.height = H; o.min_disp = min_d; o.max_disp = min_d + range;
-        o.num_layers = layers; o.do_lr_check = opt.do_lr_check;
-        o.do_rm_peaks = false; o.do_smooth = false;
-        if (!fs.Init(o)) {
+        bool ok;
+        if (depth) {
+            FasterStereoCuda::ste_opt2_t o;
+            o.width = W; o.height = H; o.min_depth = zrange[0]; o.max_depth = zrange[1];
+            o.num_layers = layers; o.do_lr_check = opt.do_lr_check;
+            o.do_rm_peaks = false; o.do_smooth = false;
+            o.epi.x0_left = cam[0]; o.epi.x0_right = cam[1];
+            o.epi.focus = cam[2]; o.epi.baseline = cam[3];
+            ok = fs.Init2(o);
+        } else {
+            FasterStereoCuda::ste_opt1_t o;
+            o.width = W; o.height = H; o.min_disp = min_d; o.max_disp = min_d + range;
+            o.num_layers = layers; o.do_lr_check = opt.do_lr_check;
+            o.do_rm_peaks = false; o.do_smooth = false;
+            ok = fs.Init(o);
+        }
+        if (!ok) {
             std::fprintf(stderr, "Init failed: %s\n", fs.LastError());
             return 1;
         }
         double ms = 0.0;
         for (int r = 0; r < repeat; ++r) {
             const auto t0 = std::chrono::steady_clock::now();
-            if (!fs.Match(left.data(), right.data(), dl.data())) {
+            if (!(depth ? fs.Match2(left.data(), right.data(), dl.data())
+                        : fs.Match(left.data(), right.data(), dl.data()))) {
                 std::fprintf(stderr, "Match failed: %s\n", fs.LastError());
                 return 1;
             }
@@ -109,14 +138,19 @@ int main(int argc, char** argv) {
     }
 
     StereoCuda stereo;
-    if (!stereo.Init(W, H, min_d, range, opt, false)) {
+    CamParam_T cp{};
+    cp.x0_left = cam[0]; cp.x0_right = cam[1]; cp.focus = cam[2]; cp.baseline = cam[3];
+    if (!(depth ? stereo.Init2(W, H, min_d, range, opt, cp, false)
+                : stereo.Init(W, H, min_d, range, opt, false))) {
         std::fprintf(stderr, "Init failed: %s\n", stereo.LastError());
         return 1;
     }
     double total_ms = 0.0;
     for (int r = 0; r < repeat; ++r) {
         const auto t0 = std::chrono::steady_clock::now();
-        if (!stereo.Match(left.data(), right.data(), dl.data(), out_right ? dr.data() : nullptr)) {
+        if (!(depth ? stereo.Match2(left.data(), right.data(), dl.data())
+                    : stereo.Match(left.data(), right.data(), dl.data(),
+                                   out_right ? dr.data() : nullptr))) {
             std::fprintf(stderr, "Match failed: %s\n", stereo.LastError());
             return 1;
         }

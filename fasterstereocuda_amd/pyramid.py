@@ -11,13 +11,13 @@ uniqueness 0.95 at level 0 and 1.0 above, LR check at the coarsest level,
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
 from . import _lib
 from ._lib import last_error, lib
-from .stereo import CuSGMOption
+from .stereo import CamParam, CuSGMOption
 
 
 class SgmPyramid:
@@ -29,6 +29,21 @@ class SgmPyramid:
         if not self._p:
             raise RuntimeError(f"sgm_pyramid_create failed: {last_error()}")
         self.width, self.height = width, height
+
+    @classmethod
+    def from_depth(cls, width: int, height: int, min_depth: float, max_depth: float,
+                   num_layers: int, level_range: int, option: CuSGMOption,
+                   cam: CamParam) -> "SgmPyramid":
+        """Pyramid whose coarsest interval comes from a depth interval
+        (HierSgmCudaImpl::Init2); match2 then returns depth."""
+        self = cls.__new__(cls)
+        opt, c = option.to_c(), cam.to_c()
+        self._p = lib().sgm_pyramid_create2(width, height, min_depth, max_depth, num_layers,
+                                            level_range, ctypes.byref(opt), ctypes.byref(c))
+        if not self._p:
+            raise RuntimeError(f"sgm_pyramid_create2 failed: {last_error()}")
+        self.width, self.height = width, height
+        return self
 
     def __del__(self):
         p = getattr(self, "_p", None)
@@ -61,6 +76,22 @@ class SgmPyramid:
         _lib.check(lib().sgm_pyramid_match_device(self._p, d_left.data_ptr(), d_right.data_ptr(),
                                                   d_disp.data_ptr(), stream))
 
+    def match2(self, left: np.ndarray, right: np.ndarray) -> np.ndarray:
+        """Depth map of a pair (pyramids made by from_depth)."""
+        left = np.ascontiguousarray(left, np.uint8)
+        right = np.ascontiguousarray(right, np.uint8)
+        out = np.empty((self.height, self.width), np.float32)
+        _lib.check(lib().sgm_pyramid_match2(self._p, left.ctypes.data, right.ctypes.data,
+                                            out.ctypes.data))
+        return out
+
+    def match2_device(self, d_left, d_right, d_depth, stream=None) -> None:
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(lib().sgm_pyramid_match2_device(self._p, d_left.data_ptr(), d_right.data_ptr(),
+                                                   d_depth.data_ptr(), stream))
+
 
 @dataclass
 class StereoOption1:
@@ -75,7 +106,36 @@ class StereoOption1:
     do_smooth: bool = False
 
 
-def reference_schedule(opt: StereoOption1) -> CuSGMOption:
+@dataclass
+class EpipolarStereoParam:
+    """epi_ste_t (FasterStereoCuda.h:26-35), same fields and defaults."""
+    x0_left: float = 0.0
+    y0_left: float = 0.0
+    x0_right: float = 0.0
+    y0_right: float = 0.0
+    baseline: float = 0.0
+    focus: float = 0.0
+
+    def camera(self) -> CamParam:
+        return CamParam(self.x0_left, self.y0_left, self.x0_right, self.y0_right, self.focus,
+                        self.baseline)
+
+
+@dataclass
+class StereoOption2:
+    """ste_opt2_t (FasterStereoCuda.h:53-66), same fields and defaults."""
+    width: int = 0
+    height: int = 0
+    min_depth: float = 0.0
+    max_depth: float = 0.0
+    num_layers: int = 1
+    do_lr_check: bool = True
+    do_rm_peaks: bool = True
+    do_smooth: bool = False
+    epi: EpipolarStereoParam = field(default_factory=EpipolarStereoParam)
+
+
+def reference_schedule(opt) -> CuSGMOption:
     """Options FasterStereoCuda.cpp:169-199 gives every level (the level
     dependent parts -- uniqueness, LR level, dilation -- are applied by the
     pyramid itself)."""
@@ -107,12 +167,38 @@ class FasterStereoCuda:
             return False
         return True
 
+    def Init2(self, option: StereoOption2) -> bool:
+        if option.width <= 0 or option.height <= 0:
+            self.last_error = "image size error"
+            return False
+        try:
+            self._pyr = SgmPyramid.from_depth(option.width, option.height, option.min_depth,
+                                              option.max_depth, max(1, option.num_layers),
+                                              self.LEVEL_RANGE, reference_schedule(option),
+                                              option.epi.camera())
+        except RuntimeError as e:
+            self.last_error = str(e)
+            self._pyr = None
+            return False
+        return True
+
     def Match(self, bytes_left: np.ndarray, bytes_right: np.ndarray, disparity: np.ndarray) -> bool:
         if self._pyr is None:
             self.last_error = "not initialised"
             return False
         try:
             disparity[...] = self._pyr.match(bytes_left, bytes_right)
+        except _lib.SgmError as e:
+            self.last_error = str(e)
+            return False
+        return True
+
+    def Match2(self, bytes_left: np.ndarray, bytes_right: np.ndarray, depth: np.ndarray) -> bool:
+        if self._pyr is None:
+            self.last_error = "not initialised"
+            return False
+        try:
+            depth[...] = self._pyr.match2(bytes_left, bytes_right)
         except _lib.SgmError as e:
             self.last_error = str(e)
             return False

@@ -13,7 +13,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import SgmOption, SgmRoi, check, lib
+from ._lib import SgmCamera, SgmOption, SgmRoi, check, lib
 
 
 @dataclass
@@ -57,6 +57,34 @@ class CuSGMOption:
 
 
 COST_AUTO, COST_STORED, COST_ON_THE_FLY = 0, 1, 2
+
+
+@dataclass
+class CamParam:
+    """CamParam_T (sgm_types.h:69-84): principal points and focal length in
+    pixels, baseline in the caller's depth unit."""
+    x0_left: float = 0.0
+    y0_left: float = 0.0
+    x0_right: float = 0.0
+    y0_right: float = 0.0
+    focus: float = 0.0
+    baseline: float = 0.0
+
+    def to_c(self) -> SgmCamera:
+        return SgmCamera(self.x0_left, self.y0_left, self.x0_right, self.y0_right, self.focus,
+                         self.baseline)
+
+
+def depth_device(d_disp, d_depth, cam: CamParam, stream=None) -> None:
+    """Depth of a device disparity map (float32 H x W tensors), enqueued on
+    `stream` (default: torch's current stream)."""
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream().cuda_stream
+    H, W = d_disp.shape
+    c = cam.to_c()
+    check(lib().sgm_depth_device(d_disp.data_ptr(), W, H, ctypes.byref(c), d_depth.data_ptr(),
+                                 stream))
 
 
 def set_cost_mode(mode: int) -> None:
@@ -119,6 +147,31 @@ class StereoCuda:
         rr = None if ste_roi_right is None else ctypes.byref(ste_roi_right)
         return self._ok(lib().sgm_match(self._h, ptr(img_left), ptr(img_right), ptr(disp_left),
                                         ptr(disp_right), init, rl, rr))
+
+    def Init2(self, width: int, height: int, min_disparity: int, disp_range: int,
+              sgm_option: CuSGMOption, cam_param: "CamParam", print_log: bool = False) -> bool:
+        """Init plus the camera Match2 converts with (StereoCuda.h:57)."""
+        opt = sgm_option.to_c()
+        cam = cam_param.to_c()
+        ok = self._ok(lib().sgm_init2(self._h, width, height, min_disparity, disp_range,
+                                      ctypes.byref(opt), ctypes.byref(cam), int(print_log)))
+        if ok:
+            self.width, self.height = width, height
+        return ok
+
+    def Match2(self, img_left: np.ndarray, img_right: np.ndarray, depth_left: np.ndarray,
+               init_disp_left=None, ste_roi_left=None, ste_roi_right=None) -> bool:
+        """Match, then the float depth of the left map into depth_left
+        (StereoCuda.h:69); invalid pixels stay -999999.0f."""
+        shape = (self.height, self.width)
+        for a, dt in ((img_left, np.uint8), (img_right, np.uint8), (depth_left, np.float32)):
+            if a.shape != shape or a.dtype != dt or not a.flags["C_CONTIGUOUS"]:
+                raise ValueError(f"expected C-contiguous {dt.__name__} array of shape {shape}")
+        init = None if init_disp_left is None else init_disp_left.ctypes.data
+        rl = None if ste_roi_left is None else ctypes.byref(ste_roi_left)
+        rr = None if ste_roi_right is None else ctypes.byref(ste_roi_right)
+        return self._ok(lib().sgm_match2(self._h, img_left.ctypes.data, img_right.ctypes.data,
+                                         depth_left.ctypes.data, init, rl, rr))
 
     def match_device(self, d_left, d_right, d_disp_left, d_disp_right=None, stream=None) -> bool:
         """Pipeline on device tensors (uint8 / float32, C-contiguous), enqueued

@@ -85,6 +85,34 @@ int sgm_match(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
 int sgm_match_device(sgm_handle* h, const uint8_t* d_left, const uint8_t* d_right,
                      float* d_disp_left, float* d_disp_right, void* stream);
 
+/* Camera of a rectified pair, field for field CamParam_T (sgm_types.h:69-84):
+ * principal points (pixels), focal length (pixels) and baseline. */
+typedef struct sgm_camera {
+    double x0_left, y0_left;
+    double x0_right, y0_right;
+    double focus;
+    double baseline;
+} sgm_camera;
+
+/* Replaces cusgm_util::ComputeDepthCuda (cusgm_util.cu:7-45):
+ *   depth = (float)(baseline * focus / ((double)disp + (x0_right - x0_left)))
+ * in float64, with the invalid value (-999999.0f) passed through.  Device
+ * pointers, packed W x H, enqueued on `stream`. */
+int sgm_depth_device(const float* d_disp, int32_t width, int32_t height, const sgm_camera* cam,
+                     float* d_depth, void* stream);
+
+/* Replaces StereoCuda::Init2 (StereoCuda.h:57, StereoCudaImpl.cpp:263-278):
+ * sgm_init, and the camera used by sgm_match2. */
+int sgm_init2(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity,
+              int32_t disp_range, const sgm_option* opt, const sgm_camera* cam, int32_t print_log);
+
+/* Replaces StereoCuda::Match2 (StereoCuda.h:69, StereoCudaImpl.cpp:548-583):
+ * the sgm_match pipeline, then the depth of the final left map (as
+ * sgm_depth_device) copied to depth_left.  Needs a handle set up by sgm_init2;
+ * init_disp_left and the ROIs must be NULL, as for sgm_match. */
+int sgm_match2(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right, float* depth_left,
+               const int16_t* init_disp_left, const sgm_roi* roi_left, const sgm_roi* roi_right);
+
 /* Replaces StereoCuda::SetMinDisparity / GetDispartyRange (StereoCuda.h:71-72). */
 int sgm_set_min_disparity(sgm_handle* h, int32_t min_disparity);
 int32_t sgm_get_disparity_range(const sgm_handle* h);
@@ -218,6 +246,25 @@ int sgm_pyramid_match(sgm_pyramid* p, const uint8_t* left, const uint8_t* right,
 /* device pointers, enqueued on stream */
 int sgm_pyramid_match_device(sgm_pyramid* p, const uint8_t* d_left, const uint8_t* d_right,
                              float* d_disp, void* stream);
+/* Replaces HierSgmCudaImpl::Init2 (FasterStereoCuda.cpp:297-406): as
+ * sgm_pyramid_create, with the coarsest level's interval taken from a depth
+ * interval.  With s = 2^(num_layers-1) and the camera scaled by 1/s
+ * (principal points and focus; CamParam_T::Scale, sgm_types.h:75-82):
+ *   min = baseline * focus / max_depth - x0_right + x0_left   (float64)
+ *   max = baseline * focus / min_depth - x0_right + x0_left
+ * the range is the smallest 64 * 2^k >= max - min (halved once if above
+ * twice the width), min moves to width - range (not below -width) when it
+ * reaches the width, and the level starts at floor(min).  The camera is kept
+ * for sgm_pyramid_match2.  0 < min_depth < max_depth, all finite. */
+sgm_pyramid* sgm_pyramid_create2(int32_t width, int32_t height, float min_depth, float max_depth,
+                                 int32_t num_layers, int32_t level_range, const sgm_option* opt,
+                                 const sgm_camera* cam);
+/* Replaces HierSgmCudaImpl::Match2 (FasterStereoCuda.cpp:531-601): the
+ * pyramid match, then the depth of the level-0 map with the unscaled camera.
+ * Needs a pyramid from sgm_pyramid_create2. */
+int sgm_pyramid_match2(sgm_pyramid* p, const uint8_t* left, const uint8_t* right, float* depth);
+int sgm_pyramid_match2_device(sgm_pyramid* p, const uint8_t* d_left, const uint8_t* d_right,
+                              float* d_depth, void* stream);
 /* range of the coarsest level (and its minimum disparity through min_disp) */
 int32_t sgm_pyramid_coarsest_range(const sgm_pyramid* p, int32_t* min_disp);
 size_t sgm_pyramid_device_bytes(const sgm_pyramid* p);

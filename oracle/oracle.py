@@ -233,6 +233,15 @@ def _plib():
         L.orc_dilation.argtypes = [P, P, I, I, I]
         L.orc_dilation_window.argtypes = [I, I]
         L.orc_dilation_window.restype = I
+        L.orc_pyramid_at.argtypes = [P, P, I, I, I, I, I, I, ctypes.POINTER(Options), P]
+        L.orc_pyramid_at.restype = I
+        D6 = ctypes.POINTER(ctypes.c_double)
+        PI = ctypes.POINTER(I)
+        L.orc_disp_interval.argtypes = [I, I, I, I, PI, PI]
+        L.orc_depth_interval.argtypes = [ctypes.c_double, ctypes.c_double, I, I, D6, PI, PI]
+        L.orc_depth_interval.restype = I
+        L.orc_round_interval.argtypes = [ctypes.c_double, I, PI, PI]
+        L.orc_depth.argtypes = [P, ctypes.c_size_t, D6, P]
         L._pyr_ready = True
     return L
 
@@ -313,4 +322,56 @@ def pyramid(left, right, min_disp, max_disp, num_layers, level_range, cs_type=1,
                              level_range, ctypes.byref(opt), _p(out))
     if rc:
         raise RuntimeError(f"orc_pyramid failed with {rc}")
+    return out
+
+
+# ------------------------------------------------------------------ depth
+def _cam6(cam):
+    """cam: (x0_left, y0_left, x0_right, y0_right, focus, baseline)"""
+    return (ctypes.c_double * 6)(*[float(v) for v in cam])
+
+
+def depth(disp, cam):
+    """Depth map of a disparity map (cusgm_util.cu:7-33)."""
+    disp = np.ascontiguousarray(disp, np.float32)
+    out = np.empty_like(disp)
+    _plib().orc_depth(_p(disp), disp.size, _cam6(cam), _p(out))
+    return out
+
+
+def disp_interval(min_disp, max_disp, W, num_layers):
+    """Coarsest (min, range) of an option-1 pyramid (FasterStereoCuda.cpp:265-281)."""
+    a, b = ctypes.c_int(), ctypes.c_int()
+    _plib().orc_disp_interval(min_disp, max_disp, W, num_layers, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def depth_interval(min_depth, max_depth, W, num_layers, cam):
+    """Coarsest (min, range) of a depth pyramid (FasterStereoCuda.cpp:373-391).
+    min_depth / max_depth are the float32 option values."""
+    a, b = ctypes.c_int(), ctypes.c_int()
+    rc = _plib().orc_depth_interval(float(np.float32(min_depth)), float(np.float32(max_depth)), W,
+                                    num_layers, _cam6(cam), ctypes.byref(a), ctypes.byref(b))
+    if rc:
+        raise ValueError("bad depth interval")
+    return a.value, b.value
+
+
+def round_interval(span, W, dmin):
+    a, b = ctypes.c_int(dmin), ctypes.c_int()
+    _plib().orc_round_interval(float(span), W, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def pyramid_at(left, right, dmin, drange, num_layers, level_range, cs_type=1, p1=10, p2=150,
+               uniqueness=0.95, do_lr_check=True, lr_threshold=1.0, do_median=True):
+    left = np.ascontiguousarray(left, np.uint8)
+    right = np.ascontiguousarray(right, np.uint8)
+    H, W = left.shape
+    opt = Options(cs_type, p1, p2, uniqueness, int(do_lr_check), lr_threshold, int(do_median))
+    out = np.empty((H, W), np.float32)
+    rc = _plib().orc_pyramid_at(_p(left), _p(right), W, H, dmin, drange, num_layers, level_range,
+                                ctypes.byref(opt), _p(out))
+    if rc:
+        raise RuntimeError(f"orc_pyramid_at failed with {rc}")
     return out

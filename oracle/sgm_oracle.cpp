@@ -514,10 +514,72 @@ int orc_match_init(const uint8_t* left, const uint8_t* right, const int16_t* ini
     return 0;
 }
 
+/* Coarsest interval of an option-1 pyramid (FasterStereoCuda.cpp:265-281):
+ * integer division by the coarsest scale, then orc_round_interval. */
+void orc_disp_interval(int min_disp, int max_disp, int W, int num_layers, int* dmin, int* range) {
+    const int scale = 1 << (num_layers - 1);
+    *dmin = min_disp / scale;
+    orc_round_interval((double)(max_disp / scale - *dmin), W, dmin, range);
+}
+
+/* FasterStereoCuda.cpp:268-281 / :377-390: range = the smallest 2^k >= span
+ * with k >= 6, halved once if above 2 W (the full width); a minimum at or
+ * past W moves to W - range, not below -W. */
+void orc_round_interval(double span, int W, int* dmin, int* range) {
+    int k = 6;
+    while (std::pow(2.0, k) < span && k < 20) ++k;
+    int r = 1 << k;
+    if (r > 2 * W) r /= 2;
+    *range = r;
+    if (*dmin >= W) {
+        *dmin = W - r;
+        if (*dmin < -W) *dmin = -W;
+    }
+}
+
+/* Coarsest interval of a depth pyramid (HierSgmCudaImpl::Init2,
+ * FasterStereoCuda.cpp:373-391): the camera scaled by s = 2^(num_layers-1)
+ * (CameraParam_T::Scale, sgm_types.h:75-82: principal points and focus
+ * divided by s), then in float64
+ *   min = baseline * focus / max_depth - x0_right + x0_left
+ *   max = baseline * focus / min_depth - x0_right + x0_left
+ * and the level starts at floor(min) (the width test is on the float64
+ * minimum, which is the same as on its floor since W is an integer). */
+int orc_depth_interval(double min_depth, double max_depth, int W, int num_layers,
+                       const double cam[6], int* dmin, int* range) {
+    if (!(min_depth > 0.0) || !(max_depth > min_depth) || num_layers < 1) return -1;
+    const int s = 1 << (num_layers - 1);
+    const double x0l = cam[0] / s, x0r = cam[2] / s, f = cam[4] / s, b = cam[5];
+    const double lo = b * f / max_depth - x0r + x0l;
+    const double hi = b * f / min_depth - x0r + x0l;
+    *dmin = (int)std::floor(lo);
+    orc_round_interval(hi - lo, W, dmin, range);
+    return 0;
+}
+
+/* Depth from disparity (cusgm_util.cu:7-33): with f1 = baseline * focus and
+ * f2 = x0_right - x0_left in float64, depth = (float)(f1 / ((double)d + f2));
+ * the invalid value passes through.  cam = {x0_left, y0_left, x0_right,
+ * y0_right, focus, baseline}. */
+void orc_depth(const float* disp, size_t n, const double cam[6], float* depth) {
+    const double f1 = cam[5] * cam[4], f2 = cam[2] - cam[0];
+    for (size_t i = 0; i < n; ++i)
+        depth[i] = disp[i] == ORC_INVALID_DISP ? ORC_INVALID_DISP : (float)(f1 / ((double)disp[i] + f2));
+}
+
 int orc_pyramid(const uint8_t* left, const uint8_t* right, int W, int H, int min_disp,
                 int max_disp, int num_layers, int level_range, const orc_options* opt,
                 float* disp) {
     if (num_layers < 1 || W <= 0 || H <= 0 || max_disp <= min_disp) return -1;
+    int dmin = 0, range = 0;
+    orc_disp_interval(min_disp, max_disp, W, num_layers, &dmin, &range);
+    return orc_pyramid_at(left, right, W, H, dmin, range, num_layers, level_range, opt, disp);
+}
+
+/* The pyramid with its coarsest level over [dmin, dmin + range). */
+int orc_pyramid_at(const uint8_t* left, const uint8_t* right, int W, int H, int dmin, int range,
+                   int num_layers, int level_range, const orc_options* opt, float* disp) {
+    if (num_layers < 1 || W <= 0 || H <= 0 || range <= 0) return -1;
     std::vector<std::vector<uint8_t>> L(num_layers), R(num_layers);
     std::vector<int> lw(num_layers), lh(num_layers);
     for (int i = 0; i < num_layers; ++i) { lw[i] = W >> i; lh[i] = H >> i; }
@@ -529,15 +591,8 @@ int orc_pyramid(const uint8_t* left, const uint8_t* right, int W, int H, int min
         orc_zoom_half(L[i - 1].data(), lw[i - 1], lh[i - 1], L[i].data());
         orc_zoom_half(R[i - 1].data(), lw[i - 1], lh[i - 1], R[i].data());
     }
-    // coarsest level: plain SGM over the scaled range, rounded up to 2^k >= 64
-    const int top = num_layers - 1, scale = 1 << top;
-    int dmin = min_disp / scale;
-    const int dmax = max_disp / scale;
-    // FasterStereoCuda.cpp:265-281 compares against the full-resolution width
-    int range = 64;
-    while (range < dmax - dmin) range <<= 1;
-    if (range > 2 * W) range >>= 1;
-    if (dmin >= W) dmin = std::max(W - range, -W);
+    // coarsest level: plain SGM over [dmin, dmin + range)
+    const int top = num_layers - 1;
     orc_options o = *opt;
     if (top > 0) o.uniqueness = 1.0f;
     std::vector<float> cur((size_t)lw[top] * lh[top]);

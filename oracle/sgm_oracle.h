@@ -19,6 +19,7 @@
  */
 #ifndef SGM_ORACLE_H_
 #define SGM_ORACLE_H_
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -88,6 +89,19 @@ int orc_match_init(const uint8_t* left, const uint8_t* right, const int16_t* ini
 int orc_pyramid(const uint8_t* left, const uint8_t* right, int W, int H, int min_disp,
                 int max_disp, int num_layers, int level_range, const orc_options* opt,
                 float* disp);
+
+/* The same pyramid with an explicit coarsest interval [dmin, dmin + range). */
+int orc_pyramid_at(const uint8_t* left, const uint8_t* right, int W, int H, int dmin, int range,
+                   int num_layers, int level_range, const orc_options* opt, float* disp);
+/* Coarsest interval from a disparity interval (FasterStereoCuda.cpp:265-281),
+ * from a depth interval and camera (:373-391; cam = {x0_left, y0_left,
+ * x0_right, y0_right, focus, baseline}), and the shared power-of-two rounding. */
+void orc_disp_interval(int min_disp, int max_disp, int W, int num_layers, int* dmin, int* range);
+int orc_depth_interval(double min_depth, double max_depth, int W, int num_layers,
+                       const double cam[6], int* dmin, int* range);
+void orc_round_interval(double span, int W, int* dmin, int* range);
+/* Depth from disparity in float64 (cusgm_util.cu:7-33), invalid passed through. */
+void orc_depth(const float* disp, size_t n, const double cam[6], float* depth);
 
 /* Whole plain pipeline (StereoCudaImpl::Match order).  disp_r may be NULL.
  * Returns 0 on success. */

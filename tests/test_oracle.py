@@ -450,3 +450,62 @@ def test_pyramid_recovers_ground_truth():
     assert valid.mean() > 0.6
     err = np.abs(d[valid] - 0.5 - p.disparity[valid])
     assert (err > 2).mean() < 0.1
+
+
+# ------------------------------------------------------------------ depth
+# Camera tuple order: (x0_left, y0_left, x0_right, y0_right, focus, baseline).
+def test_depth_closed_forms():
+    """cusgm_util.cu:7-33 pinned by values that need no retyped formula:
+    b*f / (d + x0r - x0l) at denominators 10, 1, 0, -10.5; invalid passes."""
+    cam = (510.5, 3.0, 500.0, 3.0, 1000.0, 100.0)           # b*f = 1e5, x0r - x0l = -10.5
+    d = np.array([20.5, 11.5, 10.5, 0.0, oracle.INVALID, 110.5], np.float32)
+    z = oracle.depth(d, cam)
+    assert z[0] == np.float32(10000.0)
+    assert z[1] == np.float32(100000.0)
+    assert np.isposinf(z[2])                                # zero denominator
+    assert z[3] == np.float32(-100000.0 / 10.5)
+    assert z[4] == oracle.INVALID
+    assert z[5] == np.float32(1000.0)
+    # y0 does not enter; the depth of a disparity map keeps its shape
+    cam_y = (510.5, -7.0, 500.0, 11.0, 1000.0, 100.0)
+    np.testing.assert_array_equal(oracle.depth(d, cam_y).view(np.uint32), z.view(np.uint32))
+    m = np.arange(12, dtype=np.float32).reshape(3, 4) + 11.0
+    assert oracle.depth(m, cam).shape == (3, 4)
+
+
+def test_depth_inverse_relation_and_order():
+    """depth * (d + x0r - x0l) = b * f up to one float32 rounding, and depth
+    falls as disparity grows (positive denominators)."""
+    rng = np.random.default_rng(7)
+    cam = (612.5, 0.0, 598.25, 0.0, 721.5, 0.54)
+    d = np.sort(rng.uniform(15.0, 300.0, 5000).astype(np.float32))
+    z = oracle.depth(d, cam).astype(np.float64)
+    prod = z * (d.astype(np.float64) + (598.25 - 612.5))
+    np.testing.assert_allclose(prod, 721.5 * 0.54, rtol=2.0 ** -23)
+    assert np.all(np.diff(z) <= 0)
+
+
+@pytest.mark.parametrize("args,expect", [
+    # (min_depth, max_depth, W, layers, cam) -> (floor(min), range); worked by hand
+    ((500, 2000, 1000, 1, (500, 0, 500, 0, 1000, 100)), (50, 256)),     # [50, 200]: span 150
+    ((500, 2000, 1000, 3, (500, 0, 500, 0, 1000, 100)), (12, 64)),      # f/4: [12.5, 50]
+    ((500, 2000, 1000, 1, (520, 0, 500, 0, 1000, 100)), (70, 256)),     # + x0l - x0r = 20
+    ((500, 2000, 1000, 1, (400.5, 0, 500, 0, 1000, 100)), (-50, 256)),  # 50 - 99.5 -> floor
+    ((500, 2000, 100, 1, (500, 0, 500, 0, 1000, 100)), (50, 128)),      # 256 > 2W: halved
+    ((1000 / 7.5, 1000 / 7.0, 600, 1, (0, 0, 0, 0, 10000, 10)), (536, 64)),  # min >= W: W - range
+])
+def test_depth_interval_hand_cases(args, expect):
+    """FasterStereoCuda.cpp:373-391 (with CamParam_T::Scale) on cases whose
+    outcome follows from the arithmetic by hand."""
+    assert oracle.depth_interval(*args) == expect
+
+
+def test_disp_interval_cases():
+    """FasterStereoCuda.cpp:265-281: integer scale, then the same rounding."""
+    assert oracle.disp_interval(0, 256, 1000, 3) == (0, 64)
+    assert oracle.disp_interval(0, 256, 1000, 1) == (0, 256)
+    assert oracle.disp_interval(10, 300, 1000, 2) == (5, 256)           # [5, 150): span 145
+    assert oracle.disp_interval(2000, 2100, 1000, 1) == (872, 128)      # span 100; min >= W
+    assert oracle.round_interval(64.0, 500, 0) == (0, 64)               # exactly 2^6
+    assert oracle.round_interval(64.5, 500, 0) == (0, 128)
+    assert oracle.round_interval(600.0, 200, 0) == (0, 512)             # 1024 > 400 -> 512
