@@ -88,6 +88,9 @@ SIGNATURES = {
                                  ctypes.POINTER(SgmCamera), I32]),
     "sgm_match2": (ctypes.c_int, [P, P, P, P, P, P, P]),
     "sgm_set_min_disparity": (ctypes.c_int, [P, I32]),
+    "sgm_set_peak_strips": (ctypes.c_int, [I32]),
+    "sgm_get_peak_strips": (I32, []),
+    "sgm_stage_remove_peaks": (ctypes.c_int, [P, I32, I32, I32, F32, P, P]),
     "sgm_get_disparity_range": (I32, [P]),
     "sgm_release": (ctypes.c_int, [P]),
     "sgm_last_error": (ctypes.c_char_p, [P]),

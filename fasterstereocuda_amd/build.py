@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["sgm_kernels.hip", "sgm_wide.hip", "sgm_capi.hip", "dsgm_kernels.hip",
-               "dsgm_capi.hip", "pyr_kernels.hip", "pyr_capi.hip"]
+               "dsgm_capi.hip", "pyr_kernels.hip", "pyr_capi.hip", "peaks.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

@@ -22,7 +22,7 @@ inline Impl* I(void* p) { return static_cast<Impl*>(p); }
 
 constexpr int kLevelRange = 32;   // window of the finer levels (reference LEVEL_RANGE)
 
-sgm_option level_options(bool do_lr_check, bool do_rm_peaks, bool do_smooth) {
+sgm_option level_options(bool do_lr_check, bool do_rm_peaks, bool do_smooth, int num_layers) {
     sgm_option c{};
     c.p1 = 8;
     c.p2_init = 32;
@@ -35,7 +35,8 @@ sgm_option level_options(bool do_lr_check, bool do_rm_peaks, bool do_smooth) {
     c.unique_threshold = 0.95f;
     c.using_constant_p2 = 1;
     c.do_remove_peaks = do_rm_peaks ? 1 : 0;
-    c.peaks_ratio_threshold = 0.0005f;
+    // applied at the coarsest level only; its ratio by level count (:182-196)
+    c.peaks_ratio_threshold = num_layers <= 1 ? 0.0005f : (num_layers <= 2 ? 0.001f : 0.005f);
     c.post_filter_type = do_smooth ? 1 : 0;
     c.morphology_type = 0;
     return c;
@@ -64,7 +65,8 @@ bool FasterStereoCuda::Init(const ste_opt1_t& option) const {
         m->err = "image size error";
         return false;
     }
-    const sgm_option c = level_options(option.do_lr_check, option.do_rm_peaks, option.do_smooth);
+    const sgm_option c = level_options(option.do_lr_check, option.do_rm_peaks, option.do_smooth,
+                                       std::max(1, option.num_layers));
     m->pyr = sgm_pyramid_create(option.width, option.height, option.min_disp, option.max_disp,
                                 std::max(1, option.num_layers), kLevelRange, &c);
     if (!m->pyr) {
@@ -82,7 +84,8 @@ bool FasterStereoCuda::Init2(const ste_opt2_t& option) const {
         m->err = "image size error";
         return false;
     }
-    const sgm_option c = level_options(option.do_lr_check, option.do_rm_peaks, option.do_smooth);
+    const sgm_option c = level_options(option.do_lr_check, option.do_rm_peaks, option.do_smooth,
+                                       std::max(1, option.num_layers));
     const sgm_camera cam{option.epi.x0_left,  option.epi.y0_left, option.epi.x0_right,
                          option.epi.y0_right, option.epi.focus,   option.epi.baseline};
     m->pyr = sgm_pyramid_create2(option.width, option.height, option.min_depth, option.max_depth,

@@ -57,6 +57,7 @@ struct sgm_pyramid {
     std::vector<void*> lvl_block;
     bool has_cam = false;         // set by sgm_pyramid_create2 (level-0 camera)
     sgm_camera cam{};
+    int* peaks = nullptr;         // peak-removal scratch of the coarsest level
 };
 
 namespace {
@@ -66,6 +67,7 @@ void free_pyramid(sgm_pyramid* p) {
     if (p->stream) { (void)hipStreamSynchronize(p->stream); (void)hipStreamDestroy(p->stream); }
     if (p->arena) (void)hipFree(p->arena);
     if (p->top_block) (void)hipFree(p->top_block);
+    if (p->peaks) (void)hipFree(p->peaks);
     for (void* b : p->lvl_block)
         if (b) (void)hipFree(b);
     delete p;
@@ -105,10 +107,14 @@ bool pyramid_args_ok(int32_t width, int32_t height, int32_t num_layers, int32_t 
         set_global_error(SGM_E_INVALID_ARG, "pyramid: bad options");
         return false;
     }
-    if (opt->do_remove_peaks || opt->post_filter_type != 0 || opt->morphology_type != 0 ||
-        !opt->using_constant_p2) {
+    if (opt->post_filter_type != 0 || opt->morphology_type != 0 || !opt->using_constant_p2) {
         set_global_error(SGM_E_UNSUPPORTED,
-                         "pyramid: remove-peaks, post filters, morphology and adaptive P2 are not built");
+                         "pyramid: post filters, morphology and adaptive P2 are not built");
+        return false;
+    }
+    if (opt->do_remove_peaks &&
+        !(opt->peaks_ratio_threshold >= 0.0f && opt->peaks_ratio_threshold <= 1.0f)) {
+        set_global_error(SGM_E_INVALID_ARG, "pyramid: peaks_ratio_threshold must be in [0, 1]");
         return false;
     }
     return true;
@@ -155,8 +161,11 @@ sgm_pyramid* build_pyramid(int32_t width, int32_t height, int32_t num_layers, in
         if (i == top) total += (lr ? 2 : 1) * 5 * a256(npix * range) + a256(npix * 4);
         else total += a256(npix * 2) + 5 * a256(npix * level_range);
     }
+    // peaks are removed at the coarsest level only (FasterStereoCuda.cpp:182-196)
+    const size_t top_pix = (size_t)p->lw[top] * p->lh[top];
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&p->arena, total) != hipSuccess) {
+        hipMalloc(&p->arena, total) != hipSuccess ||
+        (opt->do_remove_peaks && hipMalloc(&p->peaks, 2 * top_pix * sizeof(int)) != hipSuccess)) {
         set_global_error(SGM_E_HIP, "pyramid: device allocation failed");
         free_pyramid(p);
         return nullptr;
@@ -298,13 +307,19 @@ int run_pyramid(sgm_pyramid* p, hipStream_t s) {
         PYR_TRY(pyr_launch_zoom(z, s));
     }
     PYR_TRY(launch_match(p->top_plan, false, s, nullptr));
+    // the coarsest level of a multi-level pyramid is dilated
+    // (FasterStereoCuda.cpp:192-197, cusgm_df.cu:300-318); its peaks are then
+    // removed if asked, after the filters as in StereoCudaImpl::Match (:541-542)
+    float* top_map = p->out[top];
     if (top > 0) {
-        // the coarsest level of a multi-level pyramid is dilated
-        // (FasterStereoCuda.cpp:192-197, cusgm_df.cu:300-318)
         const int w = p->lw[top], h = p->lh[top];
         const int sm = std::min(w, h);
         PYR_TRY(pyr_launch_dilation(p->out[top], p->raw[top], w, h, sm < 360 ? 3 : (sm < 720 ? 5 : 7), s));
+        top_map = p->raw[top];
     }
+    if (p->opt.do_remove_peaks)
+        PYR_TRY(launch_remove_peaks(top_map, p->peaks, p->lw[top], p->lh[top], peak_strips(),
+                                    p->opt.peaks_ratio_threshold, s));
     for (int i = top - 1; i >= 0; --i) {
         PyrInit in{top > 0 && i == top - 1 ? p->raw[top] : p->out[i + 1], p->init[i], p->lw[i], p->lh[i], p->LR};
         PYR_TRY(pyr_launch_init_next(in, s));

@@ -49,8 +49,10 @@ int set_global_error(int code, const std::string& msg) { return fail(nullptr, co
 namespace {
 std::atomic<int> g_cost_mode{0};
 std::atomic<int> g_wide_min_groups{-1};
+std::atomic<int> g_peak_strips{1};
 }  // namespace
 int cost_mode() { return g_cost_mode.load(); }
+int peak_strips() { return g_peak_strips.load(); }
 
 // Horizontal scanline groups a plan needs before SGM_COST_AUTO picks the
 // on-the-fly kernels: two waves per SIMD of the current device.  With fewer,
@@ -83,14 +85,17 @@ inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 bool valid_range(int D) { return D == 64 || D == 128 || D == 256 || D == 512; }
 
 // Validates the option fields the engine reads; returns SGM_OK or an error.
-int check_option(const sgm_option* o, std::string* err) {
+// allow_peaks: the plain handle removes peaks; the batch does not
+int check_option(const sgm_option* o, std::string* err, bool allow_peaks) {
     if (!o) return fail(err, SGM_E_INVALID_ARG, "option pointer is NULL");
     if (o->cs_type != 0 && o->cs_type != 1)
         return fail(err, SGM_E_INVALID_ARG, "cs_type must be 0 (5x5) or 1 (9x7)");
     if (o->p1 < 0 || o->p2_init < 0 || (int64_t)o->p1 + o->p2_init > 65000)
         return fail(err, SGM_E_INVALID_ARG, "P1 and P2 must be >= 0 with P1 + P2 <= 65000");
-    if (o->do_remove_peaks)
-        return fail(err, SGM_E_UNSUPPORTED, "do_remove_peaks is not built");
+    if (o->do_remove_peaks && !allow_peaks)
+        return fail(err, SGM_E_UNSUPPORTED, "do_remove_peaks is not built for batches");
+    if (o->do_remove_peaks && !(o->peaks_ratio_threshold >= 0.0f && o->peaks_ratio_threshold <= 1.0f))
+        return fail(err, SGM_E_INVALID_ARG, "peaks_ratio_threshold must be in [0, 1]");
     if (o->post_filter_type != 0)
         return fail(err, SGM_E_UNSUPPORTED, "post filters (Gauss/bilateral) are not built");
     if (o->morphology_type != 0)
@@ -263,6 +268,7 @@ struct sgm_handle {
     StagePlan plan;
     bool has_cam = false;         // set by sgm_init2, used by sgm_match2
     sgm_camera cam{};
+    int* peaks = nullptr;         // 2 W H ints when the option removes peaks
     std::string err;
 };
 
@@ -330,13 +336,23 @@ int build_handle_plan(sgm_handle* h) {
 // right view is still censused (the left cost needs it) but not aggregated.
 int run_pipeline(sgm_handle* h, bool want_right, hipStream_t s) {
     const bool lr = h->opt.do_lr_check != 0;
+    // peak removal on the final left map, as StereoCudaImpl::Match does after
+    // the filters (StereoCudaImpl.cpp:541-542)
+    auto peaks = [&]() {
+        return h->opt.do_remove_peaks
+                   ? launch_remove_peaks(h->view[0].out, h->peaks, h->W, h->H, peak_strips(),
+                                         h->opt.peaks_ratio_threshold, s)
+                   : hipSuccess;
+    };
     if (!h->print_log) {
         HIP_TRY(&h->err, launch_match(h->plan, want_right && lr, s, nullptr));
+        HIP_TRY(&h->err, peaks());
         return SGM_OK;
     }
     hipEvent_t ev[6];
     for (auto& e : ev) HIP_TRY(&h->err, hipEventCreate(&e));
     HIP_TRY(&h->err, launch_match(h->plan, want_right && lr, s, ev));
+    HIP_TRY(&h->err, peaks());
     HIP_TRY(&h->err, hipEventSynchronize(ev[5]));
     static const char* names[5] = {"Census:", "Cost:", "Aggregate (3 paths):",
                                    "Aggregate + WTA:", "LRCheck/Filter:"};
@@ -378,7 +394,7 @@ int sgm_init(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity
         return fail(&h->err, SGM_E_INVALID_ARG, "disp_range must be 64, 128, 256 or 512");
     if (std::max(0, min_disparity) >= width)
         return fail(&h->err, SGM_E_INVALID_ARG, "min_disparity leaves no column to match");
-    if (int rc = check_option(opt, &h->err)) return rc;
+    if (int rc = check_option(opt, &h->err, true)) return rc;
     sgm_release(h);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -389,6 +405,8 @@ int sgm_init(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity
     h->print_log = print_log != 0;
     HIP_TRY(&h->err, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIP_TRY(&h->err, hipMalloc(&h->arena, arena_bytes(width, height, disp_range)));
+    if (opt->do_remove_peaks)
+        HIP_TRY(&h->err, hipMalloc(&h->peaks, 2 * (size_t)width * height * sizeof(int)));
     carve(h);
     if (int rc = build_handle_plan(h)) return rc;
     h->inited = true;
@@ -502,6 +520,7 @@ int sgm_release(sgm_handle* h) {
     }
     if (h->arena) { (void)hipFree(h->arena); h->arena = nullptr; }
     if (h->plan_block) { (void)hipFree(h->plan_block); h->plan_block = nullptr; }
+    if (h->peaks) { (void)hipFree(h->peaks); h->peaks = nullptr; }
     for (auto& v : h->view) v = ViewBuffers{};
     h->plan = StagePlan{};
     h->inited = false;
@@ -561,6 +580,15 @@ int sgm_set_cost_mode(int32_t mode) {
 }
 
 int32_t sgm_get_cost_mode(void) { return fsgm::cost_mode(); }
+
+int sgm_set_peak_strips(int32_t strips) {
+    if (strips < 1 || strips > 4096)
+        return fail(nullptr, SGM_E_INVALID_ARG, "peak strips must be in 1..4096");
+    fsgm::g_peak_strips.store(strips);
+    return SGM_OK;
+}
+
+int32_t sgm_get_peak_strips(void) { return fsgm::peak_strips(); }
 
 float sgm_invalid_float(void) { return SGM_INVALID_FLOAT; }
 int16_t sgm_invalid_short(void) { return SGM_INVALID_SHORT; }
@@ -755,7 +783,7 @@ extern "C" {
 sgm_batch* sgm_batch_create(const sgm_pair_desc* pairs, int32_t n, const sgm_option* opt) {
     std::string err;
     if (!pairs || n <= 0) { fail(nullptr, SGM_E_INVALID_ARG, "empty batch"); return nullptr; }
-    if (check_option(opt, &err)) return nullptr;
+    if (check_option(opt, &err, false)) return nullptr;
     for (int i = 0; i < n; ++i) {
         const sgm_pair_desc& p = pairs[i];
         if (p.width <= 0 || p.height <= 0 || !valid_range(p.disp_range) ||
@@ -1116,6 +1144,17 @@ int sgm_stage_post(const float* d_raw_left, const float* d_raw_right, int32_t wi
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIP_TRY(nullptr, launch_post(t.plan, s));
     if (d_out_right && do_lr_check) HIP_TRY(nullptr, launch_right_invalidate(t.plan, s));
+    HIP_TRY(nullptr, hipStreamSynchronize(s));
+    return SGM_OK;
+}
+
+int sgm_stage_remove_peaks(float* d_disp, int32_t width, int32_t height, int32_t strips,
+                           float ratio, int32_t* d_scratch, void* stream) {
+    if (!d_disp || !d_scratch || width <= 0 || height <= 0 || strips < 1 ||
+        (int64_t)width * height > INT32_MAX || !(ratio >= 0.0f && ratio <= 1.0f))
+        return fail(nullptr, SGM_E_INVALID_ARG, "remove_peaks: bad arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(nullptr, launch_remove_peaks(d_disp, d_scratch, width, height, strips, ratio, s));
     HIP_TRY(nullptr, hipStreamSynchronize(s));
     return SGM_OK;
 }

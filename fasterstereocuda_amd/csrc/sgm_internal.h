@@ -154,6 +154,12 @@ hipError_t launch_wta(const StagePlan& p, hipStream_t s);
 hipError_t launch_post(const StagePlan& p, hipStream_t s);
 hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s);
 hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t s);
+// peaks.hip: segments of fewer than (int)((float)(W r) ratio) pixels of each
+// r = H / strips row strip become invalid (in place); scratch = 2 W H ints
+hipError_t launch_remove_peaks(float* disp, int* scratch, int W, int H, int strips, float ratio,
+                               hipStream_t s);
+int peaks_threshold(int W, int strip_rows, float ratio);
+int peak_strips();   // process-wide strip count (sgm_set_peak_strips)
 // depth = (float)(f1 / ((double)disp + f2)), invalid kept (n pixels)
 hipError_t launch_depth(const float* disp, float* depth, size_t n, double f1, double f2,
                         hipStream_t s);

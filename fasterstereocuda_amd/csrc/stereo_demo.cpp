@@ -4,7 +4,7 @@
 //   stereo_demo LEFT.u8 RIGHT.u8 W H MIN_D RANGE OUT_LEFT.f32 [OUT_RIGHT.f32]
 //               [--cs 0|1] [--p1 N] [--p2 N] [--uniq F] [--lr 0|1] [--lrthr F]
 //               [--median 0|1] [--repeat N] [--layers N]
-//               [--cam X0L,X0R,FOCUS,BASELINE] [--zrange MIN,MAX]
+//               [--cam X0L,X0R,FOCUS,BASELINE] [--zrange MIN,MAX] [--peaks RATIO]
 //
 // Without --layers: StereoCuda (single level, the options above).  With
 // --layers N: FasterStereoCuda (N-level pyramid over [MIN_D, MIN_D + RANGE),
@@ -12,6 +12,8 @@
 // With --cam the output is depth instead of disparity, through Init2 /
 // Match2: StereoCuda keeps MIN_D and RANGE; FasterStereoCuda takes its
 // interval from --zrange (the depth interval) and ignores MIN_D and RANGE.
+// --peaks turns on peak removal: StereoCuda with that ratio, FasterStereoCuda
+// with do_rm_peaks (its own ratio; any RATIO value).
 // Inputs are tightly packed uint8 W x H; outputs tightly packed float32.
 // Prints the average Match time over --repeat calls after the first.
 #include <chrono>
@@ -79,6 +81,10 @@ int main(int argc, char** argv) {
         else if (k == "--median") opt.do_median_filter = std::atoi(v) != 0;
         else if (k == "--repeat") repeat = std::atoi(v);
         else if (k == "--layers") layers = std::atoi(v);
+        else if (k == "--peaks") {
+            opt.do_remove_peaks = true;
+            opt.peaks_ratio_threshold = (float)std::atof(v);
+        }
         else if (k == "--cam") {
             depth = std::sscanf(v, "%lf,%lf,%lf,%lf", &cam[0], &cam[1], &cam[2], &cam[3]) == 4;
             if (!depth) { std::fprintf(stderr, "--cam needs X0L,X0R,FOCUS,BASELINE\n"); return 2; }
@@ -106,7 +112,7 @@ int main(int argc, char** argv) {
             FasterStereoCuda::ste_opt2_t o;
             o.width = W; o.height = H; o.min_depth = zrange[0]; o.max_depth = zrange[1];
             o.num_layers = layers; o.do_lr_check = opt.do_lr_check;
-            o.do_rm_peaks = false; o.do_smooth = false;
+            o.do_rm_peaks = opt.do_remove_peaks; o.do_smooth = false;
             o.epi.x0_left = cam[0]; o.epi.x0_right = cam[1];
             o.epi.focus = cam[2]; o.epi.baseline = cam[3];
             ok = fs.Init2(o);
@@ -114,7 +120,7 @@ int main(int argc, char** argv) {
             FasterStereoCuda::ste_opt1_t o;
             o.width = W; o.height = H; o.min_disp = min_d; o.max_disp = min_d + range;
             o.num_layers = layers; o.do_lr_check = opt.do_lr_check;
-            o.do_rm_peaks = false; o.do_smooth = false;
+            o.do_rm_peaks = opt.do_remove_peaks; o.do_smooth = false;
             ok = fs.Init(o);
         }
         if (!ok) {

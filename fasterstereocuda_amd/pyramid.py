@@ -137,12 +137,15 @@ class StereoOption2:
 
 def reference_schedule(opt) -> CuSGMOption:
     """Options FasterStereoCuda.cpp:169-199 gives every level (the level
-    dependent parts -- uniqueness, LR level, dilation -- are applied by the
-    pyramid itself)."""
+    dependent parts -- uniqueness, LR level, dilation, where peaks are
+    removed -- are applied by the pyramid itself).  Peaks go at the coarsest
+    level only, with ratio 0.0005 / 0.001 / 0.005 for 1 / 2 / more levels."""
+    layers = max(1, opt.num_layers)
+    ratio = 0.0005 if layers <= 1 else (0.001 if layers <= 2 else 0.005)
     return CuSGMOption(p1=8, p2_init=32, p2_max=64, cs_type=CuSGMOption.CS_5x5, num_paths=4,
                        do_median_filter=True, do_lr_check=opt.do_lr_check, lr_threshold=1.0,
                        unique_threshold=0.95, using_constant_p2=True,
-                       do_remove_peaks=opt.do_rm_peaks, peaks_ratio_threshold=0.0005,
+                       do_remove_peaks=opt.do_rm_peaks, peaks_ratio_threshold=ratio,
                        post_filter_type=1 if opt.do_smooth else 0)
 
 

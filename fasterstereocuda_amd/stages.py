@@ -76,6 +76,17 @@ def post(raw_left: torch.Tensor, raw_right: torch.Tensor | None, do_lr: bool, lr
     return out_l, out_r
 
 
+def remove_peaks(disp: torch.Tensor, strips: int = 1, ratio: float = 0.0005) -> torch.Tensor:
+    """Peak removal of a device disparity map; returns a new map."""
+    _dev(disp, torch.float32)
+    H, W = disp.shape
+    out = disp.clone()
+    scratch = torch.empty(2 * H * W, dtype=torch.int32, device=disp.device)
+    check(lib().sgm_stage_remove_peaks(out.data_ptr(), W, H, strips, ratio, scratch.data_ptr(),
+                                       _stream()))
+    return out
+
+
 # ---- coarse-to-fine pyramid stages -------------------------------------
 def zoom_half(img: torch.Tensor) -> torch.Tensor:
     _dev(img, torch.uint8)

@@ -87,6 +87,16 @@ def depth_device(d_disp, d_depth, cam: CamParam, stream=None) -> None:
                                  stream))
 
 
+def set_peak_strips(strips: int) -> None:
+    """Row strips of do_remove_peaks (process-wide, default 1); the
+    reference uses its host's hardware_concurrency() * 3 // 4."""
+    check(lib().sgm_set_peak_strips(strips))
+
+
+def get_peak_strips() -> int:
+    return lib().sgm_get_peak_strips()
+
+
 def set_cost_mode(mode: int) -> None:
     """Where the aggregation takes its matching cost, for handles, batches
     and pyramids created afterwards: COST_ON_THE_FLY recomputes it from the

@@ -50,8 +50,9 @@ typedef struct sgm_option {
     float   lr_threshold;
     float   unique_threshold;
     int32_t using_constant_p2;      /* only constant P2 is built */
-    int32_t do_remove_peaks;        /* must be 0 (not built) */
-    float   peaks_ratio_threshold;
+    int32_t do_remove_peaks;        /* left-map peak removal (handles and pyramids;
+                                       batches refuse it); see sgm_set_peak_strips */
+    float   peaks_ratio_threshold;  /* in [0, 1] */
     int32_t post_filter_type;       /* must be 0 = PF_NONE */
     int32_t morphology_type;        /* must be 0 = MP_NONE */
 } sgm_option;
@@ -148,6 +149,18 @@ void sgm_roi_from_disparity(const float* disp, int32_t width, int32_t height, sg
 int sgm_set_cost_mode(int32_t mode);
 int32_t sgm_get_cost_mode(void);
 
+/* Row strips of the peak removal (do_remove_peaks; StereoCudaImpl.cpp:32-97,
+ * 463-498).  The reference splits the left map into
+ * hardware_concurrency() * 3 / 4 strips of H / strips rows (rows past the last
+ * strip are left alone) and removes, per strip, the 4-connected segments of
+ * valid pixels (neighbours joined when |d - d'| <= 1) smaller than
+ * (int)((float)(W * rows_per_strip) * peaks_ratio_threshold) pixels.  The
+ * strip count therefore depends on the reference's host; here it is this
+ * process-wide value, default 1 (the whole image).  Set it to the reference
+ * host's count to reproduce that host.  Read at every match. */
+int sgm_set_peak_strips(int32_t strips);
+int32_t sgm_get_peak_strips(void);
+
 /* StereoCuda::get_invad_float / get_invad_short / get_level_range
  * (StereoCuda.h:87-91). */
 float sgm_invalid_float(void);
@@ -237,7 +250,12 @@ int sgm_dsgm_backward(const float* d_cost, const float* d_p1, const float* d_p2,
  * [min/2^i, max/2^i) rounded up to 2^k >= 64 (LR check there if
  * opt->do_lr_check); each finer level searches level_range (16 or 32)
  * disparities around 2 d_coarse - level_range/2.  opt->unique_threshold
- * applies at level 0, 1.0 above.  NULL on failure (sgm_last_error(NULL)). */
+ * applies at level 0, 1.0 above.  With opt->do_remove_peaks the coarsest
+ * level's map loses its peaks after its filters (and after its dilation when
+ * there are finer levels), with opt->peaks_ratio_threshold as given; the
+ * reference's schedule (FasterStereoCuda.cpp:182-196) removes them only there,
+ * with ratio 0.0005 / 0.001 / 0.005 for 1 / 2 / more levels.  NULL on
+ * failure (sgm_last_error(NULL)). */
 typedef struct sgm_pyramid sgm_pyramid;
 sgm_pyramid* sgm_pyramid_create(int32_t width, int32_t height, int32_t min_disp, int32_t max_disp,
                                 int32_t num_layers, int32_t level_range, const sgm_option* opt);
@@ -298,6 +316,12 @@ int sgm_stage_wta(uint8_t* const d_paths[4], int32_t width, int32_t height,
 int sgm_stage_post(const float* d_raw_left, const float* d_raw_right, int32_t width,
                    int32_t height, int32_t do_lr_check, float lr_threshold,
                    int32_t do_median, float* d_out_left, float* d_out_right, void* stream);
+
+/* StereoCudaImpl.cpp:32-97, 463-498: peak removal of a device map in place
+ * (as with do_remove_peaks; strips as in sgm_set_peak_strips).  d_scratch
+ * holds 2 * width * height ints. */
+int sgm_stage_remove_peaks(float* d_disp, int32_t width, int32_t height, int32_t strips,
+                           float ratio, int32_t* d_scratch, void* stream);
 
 /* FasterStereoCuda.cpp:114-136: d_dst[(h/2)][(w/2)] = d_src[2i][2j]. */
 int sgm_stage_zoom(const uint8_t* d_src, int32_t w, int32_t h, uint8_t* d_dst, void* stream);

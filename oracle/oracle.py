@@ -31,6 +31,9 @@ class Options(ctypes.Structure):
         ("do_lr_check", ctypes.c_int32),
         ("lr_threshold", ctypes.c_float),
         ("do_median", ctypes.c_int32),
+        ("do_remove_peaks", ctypes.c_int32),
+        ("peaks_ratio", ctypes.c_float),
+        ("peak_strips", ctypes.c_int32),
     ]
 
 
@@ -125,12 +128,15 @@ def median3x3(disp: np.ndarray) -> np.ndarray:
 
 def match(left: np.ndarray, right: np.ndarray, min_d: int, D: int, cs_type: int = 1,
           p1: int = 10, p2: int = 150, uniqueness: float = 0.95, do_lr_check: bool = True,
-          lr_threshold: float = 1.0, do_median: bool = True):
-    """Whole plain pipeline.  Returns (disp_left, disp_right)."""
+          lr_threshold: float = 1.0, do_median: bool = True, peaks_ratio: float | None = None,
+          peak_strips: int = 1):
+    """Whole plain pipeline.  Returns (disp_left, disp_right).  With
+    peaks_ratio set, the left map loses its peaks (remove_peaks)."""
     left = np.ascontiguousarray(left, dtype=np.uint8)
     right = np.ascontiguousarray(right, dtype=np.uint8)
     H, W = left.shape
-    opt = Options(cs_type, p1, p2, uniqueness, int(do_lr_check), lr_threshold, int(do_median))
+    opt = Options(cs_type, p1, p2, uniqueness, int(do_lr_check), lr_threshold, int(do_median),
+                  int(peaks_ratio is not None), peaks_ratio or 0.0, peak_strips)
     dl = np.empty((H, W), np.float32)
     dr = np.empty((H, W), np.float32)
     rc = lib().orc_match(_p(left), _p(right), W, H, min_d, D, ctypes.byref(opt), _p(dl), _p(dr))
@@ -312,11 +318,13 @@ def match_init(left, right, init, level_range, cs_type=1, p1=10, p2=150, uniquen
 
 
 def pyramid(left, right, min_disp, max_disp, num_layers, level_range, cs_type=1, p1=10, p2=150,
-            uniqueness=0.95, do_lr_check=True, lr_threshold=1.0, do_median=True):
+            uniqueness=0.95, do_lr_check=True, lr_threshold=1.0, do_median=True,
+            peaks_ratio=None, peak_strips=1):
     left = np.ascontiguousarray(left, np.uint8)
     right = np.ascontiguousarray(right, np.uint8)
     H, W = left.shape
-    opt = Options(cs_type, p1, p2, uniqueness, int(do_lr_check), lr_threshold, int(do_median))
+    opt = Options(cs_type, p1, p2, uniqueness, int(do_lr_check), lr_threshold, int(do_median),
+                  int(peaks_ratio is not None), peaks_ratio or 0.0, peak_strips)
     out = np.empty((H, W), np.float32)
     rc = _plib().orc_pyramid(_p(left), _p(right), W, H, min_disp, max_disp, num_layers,
                              level_range, ctypes.byref(opt), _p(out))
@@ -364,14 +372,29 @@ def round_interval(span, W, dmin):
 
 
 def pyramid_at(left, right, dmin, drange, num_layers, level_range, cs_type=1, p1=10, p2=150,
-               uniqueness=0.95, do_lr_check=True, lr_threshold=1.0, do_median=True):
+               uniqueness=0.95, do_lr_check=True, lr_threshold=1.0, do_median=True,
+               peaks_ratio=None, peak_strips=1):
     left = np.ascontiguousarray(left, np.uint8)
     right = np.ascontiguousarray(right, np.uint8)
     H, W = left.shape
-    opt = Options(cs_type, p1, p2, uniqueness, int(do_lr_check), lr_threshold, int(do_median))
+    opt = Options(cs_type, p1, p2, uniqueness, int(do_lr_check), lr_threshold, int(do_median),
+                  int(peaks_ratio is not None), peaks_ratio or 0.0, peak_strips)
     out = np.empty((H, W), np.float32)
     rc = _plib().orc_pyramid_at(_p(left), _p(right), W, H, dmin, drange, num_layers, level_range,
                                 ctypes.byref(opt), _p(out))
     if rc:
         raise RuntimeError(f"orc_pyramid_at failed with {rc}")
     return out
+
+
+def remove_peaks(disp, strips=1, ratio=0.0005):
+    """Peak removal of a disparity map (StereoCudaImpl.cpp:32-97, 463-498);
+    returns (new map, pixels removed)."""
+    out = np.array(disp, np.float32, copy=True, order="C")
+    H, W = out.shape
+    L = lib()
+    L.orc_remove_peaks.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_float]
+    L.orc_remove_peaks.restype = ctypes.c_int
+    n = L.orc_remove_peaks(_p(out), W, H, strips, ratio)
+    return out, n

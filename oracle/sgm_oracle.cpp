@@ -514,6 +514,48 @@ int orc_match_init(const uint8_t* left, const uint8_t* right, const int16_t* ini
     return 0;
 }
 
+/* Peak removal (StereoCudaImpl.cpp:32-97 per strip, 463-498 for the strips
+ * and threshold): a breadth-first flood from every unlabelled valid pixel in
+ * raster order, growing across 4-neighbours of the same strip whose float
+ * difference to the pixel being expanded is at most 1 (the reference's
+ * abs(search - base) <= 1.0f on float32 values). */
+int orc_remove_peaks(float* disp, int W, int H, int strips, float ratio) {
+    if (W <= 0 || H <= 0 || strips <= 0) return 0;
+    const int rows = H / strips;
+    if (rows == 0) return 0;
+    const int threshold = (int)((float)(W * rows) * ratio);
+    int removed = 0;
+    std::vector<int> label((size_t)W * rows), queue;
+    for (int t = 0; t < strips; ++t) {
+        float* d = disp + (size_t)t * rows * W;
+        std::fill(label.begin(), label.end(), 0);
+        int next = 0;
+        for (int i = 0; i < W * rows; ++i) {
+            if (d[i] == ORC_INVALID_DISP || label[i]) continue;
+            label[i] = ++next;
+            queue.assign(1, i);
+            for (size_t k = 0; k < queue.size(); ++k) {
+                const int c = queue[k], y = c / W, x = c % W;
+                const int nb[4][2] = {{y - 1, x}, {y, x - 1}, {y, x + 1}, {y + 1, x}};
+                for (const auto& n : nb) {
+                    if (n[0] < 0 || n[0] >= rows || n[1] < 0 || n[1] >= W) continue;
+                    const int j = n[0] * W + n[1];
+                    if (label[j] || d[j] == ORC_INVALID_DISP) continue;
+                    if (std::fabs(d[j] - d[c]) <= 1.0f) {
+                        label[j] = next;
+                        queue.push_back(j);
+                    }
+                }
+            }
+            if ((int)queue.size() < threshold) {
+                for (int j : queue) d[j] = ORC_INVALID_DISP;
+                removed += (int)queue.size();
+            }
+        }
+    }
+    return removed;
+}
+
 /* Coarsest interval of an option-1 pyramid (FasterStereoCuda.cpp:265-281):
  * integer division by the coarsest scale, then orc_round_interval. */
 void orc_disp_interval(int min_disp, int max_disp, int W, int num_layers, int* dmin, int* range) {
@@ -596,7 +638,9 @@ int orc_pyramid_at(const uint8_t* left, const uint8_t* right, int W, int H, int 
     orc_options o = *opt;
     if (top > 0) o.uniqueness = 1.0f;
     std::vector<float> cur((size_t)lw[top] * lh[top]);
+    // peaks go at the coarsest level, after its filters (FasterStereoCuda.cpp:182-196)
     if (top == 0) return orc_match(left, right, W, H, dmin, range, &o, disp, nullptr);
+    o.do_remove_peaks = 0;
     int rc = orc_match(L[top].data(), R[top].data(), lw[top], lh[top], dmin, range, &o, cur.data(), nullptr);
     if (rc) return rc;
     {   // the coarsest level of a multi-level pyramid is dilated (FasterStereoCuda.cpp:192-197)
@@ -604,6 +648,8 @@ int orc_pyramid_at(const uint8_t* left, const uint8_t* right, int W, int H, int 
         orc_dilation(cur.data(), dil.data(), lw[top], lh[top], orc_dilation_window(lw[top], lh[top]));
         cur.swap(dil);
     }
+    if (opt->do_remove_peaks)
+        orc_remove_peaks(cur.data(), lw[top], lh[top], opt->peak_strips, opt->peaks_ratio);
     o.do_lr_check = 0;
     for (int n = top - 1; n >= 0; --n) {
         std::vector<int16_t> init((size_t)lw[n] * lh[n]);
@@ -659,6 +705,8 @@ int orc_match(const uint8_t* left, const uint8_t* right, int W, int H,
     } else {
         std::memcpy(disp_l, dl.data(), npix * sizeof(float));
     }
+    // after the filters, on the left map only (StereoCudaImpl.cpp:541-542)
+    if (opt->do_remove_peaks) orc_remove_peaks(disp_l, W, H, opt->peak_strips, opt->peaks_ratio);
     if (disp_r) std::memcpy(disp_r, dr.data(), npix * sizeof(float));
     return 0;
 }

@@ -42,6 +42,9 @@ typedef struct orc_options {
     int32_t do_lr_check;
     float   lr_threshold;
     int32_t do_median;
+    int32_t do_remove_peaks;  /* left map only, after the filters */
+    float   peaks_ratio;      /* CuSGMOption::peaks_ratio_threshold */
+    int32_t peak_strips;      /* row strips (the reference: its host's core count) */
 } orc_options;
 
 int  orc_num_threads(void);
@@ -64,6 +67,13 @@ void orc_lrcheck(float* disp_l, const float* disp_r_in, float* disp_r_out,
                  int W, int H, float threshold);
 
 void orc_median3x3(const float* in, float* out, int W, int H);
+
+/* Peak removal (StereoCudaImpl.cpp:32-97, 463-498), in place: per strip of
+ * H / strips rows (rows past the last strip untouched), 4-connected segments
+ * of valid pixels joined when |d - d'| <= 1 (float) smaller than
+ * (int)((float)(W * rows) * ratio) pixels become invalid.  Returns the
+ * number of pixels removed. */
+int orc_remove_peaks(float* disp, int W, int H, int strips, float ratio);
 
 /* ---- coarse-to-fine pyramid (FasterStereoCuda) ---- */
 #define ORC_INVALID_SHORT (-32768)

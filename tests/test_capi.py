@@ -105,20 +105,29 @@ def test_init_validates_before_device(args, code, needle):
     lib.sgm_destroy(h)
 
 
-@pytest.mark.parametrize("field,value,code", [
-    ("do_remove_peaks", True, _lib.SGM_E_UNSUPPORTED),
-    ("post_filter_type", 1, _lib.SGM_E_UNSUPPORTED),
-    ("morphology_type", 2, _lib.SGM_E_UNSUPPORTED),
-    ("p1", -1, _lib.SGM_E_INVALID_ARG),
-    ("p2_init", 70000, _lib.SGM_E_INVALID_ARG),
-    ("cs_type", 3, _lib.SGM_E_INVALID_ARG),
-])
-def test_option_validation(field, value, code):
+@pytest.mark.parametrize("fields,code", [
+    ({"do_remove_peaks": True, "peaks_ratio_threshold": 2.0}, _lib.SGM_E_INVALID_ARG),
+    ({"do_remove_peaks": True, "peaks_ratio_threshold": -0.1}, _lib.SGM_E_INVALID_ARG),
+    ({"post_filter_type": 1}, _lib.SGM_E_UNSUPPORTED),
+    ({"morphology_type": 2}, _lib.SGM_E_UNSUPPORTED),
+    ({"p1": -1}, _lib.SGM_E_INVALID_ARG),
+    ({"p2_init": 70000}, _lib.SGM_E_INVALID_ARG),
+    ({"cs_type": 3}, _lib.SGM_E_INVALID_ARG),
+], ids=lambda v: "-".join(f"{k}={x}" for k, x in v.items()) if isinstance(v, dict) else str(v))
+def test_option_validation(fields, code):
     lib = _lib.lib()
     h = lib.sgm_create()
-    opt = CuSGMOption.benchmark(**{field: value}).to_c()
+    opt = CuSGMOption.benchmark(**fields).to_c()
     assert lib.sgm_init(h, 100, 50, 0, 64, ctypes.byref(opt), 0) == code
     lib.sgm_destroy(h)
+
+
+def test_peak_strips_setting():
+    lib = _lib.lib()
+    assert lib.sgm_get_peak_strips() == 1
+    assert lib.sgm_set_peak_strips(0) == _lib.SGM_E_INVALID_ARG
+    assert lib.sgm_set_peak_strips(6) == 0 and lib.sgm_get_peak_strips() == 6
+    assert lib.sgm_set_peak_strips(1) == 0
 
 
 def test_uninitialised_handle_and_constants():

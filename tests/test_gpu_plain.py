@@ -206,8 +206,9 @@ def test_errors_are_reported(cuda):
     st = StereoCuda()
     assert not st.Init(100, 50, 0, 96, CuSGMOption.benchmark())
     assert "disp_range" in st.last_error
-    assert not st.Init(100, 50, 0, 64, CuSGMOption.benchmark(do_remove_peaks=True))
-    assert "remove_peaks" in st.last_error
+    assert not st.Init(100, 50, 0, 64, CuSGMOption.benchmark(do_remove_peaks=True,
+                                                            peaks_ratio_threshold=1.5))
+    assert "peaks_ratio_threshold" in st.last_error
     left = np.zeros((50, 100), np.uint8)
     out = np.empty((50, 100), np.float32)
     fresh = StereoCuda()

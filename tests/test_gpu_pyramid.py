@@ -100,6 +100,8 @@ def test_faster_stereo_cuda_mirror(cuda):
     assert fs.Match(p.left, p.right, out)
     exp = oracle.pyramid(p.left, p.right, 0, 128, 2, 32, cs_type=0, p1=8, p2=32)
     np.testing.assert_array_equal(_bits(out), _bits(exp))
+    # the defaults (peak removal on) are accepted; smoothing is outside the built scope
+    assert FasterStereoCuda().Init(StereoOption1(width=400, height=300, max_disp=128))
     bad = FasterStereoCuda()
-    assert not bad.Init(StereoOption1(width=400, height=300, max_disp=128))   # rm-peaks default on
-    assert "remove-peaks" in bad.last_error
+    assert not bad.Init(StereoOption1(width=400, height=300, max_disp=128, do_smooth=True))
+    assert "post filters" in bad.last_error

@@ -509,3 +509,97 @@ def test_disp_interval_cases():
     assert oracle.round_interval(64.0, 500, 0) == (0, 64)               # exactly 2^6
     assert oracle.round_interval(64.5, 500, 0) == (0, 128)
     assert oracle.round_interval(600.0, 200, 0) == (0, 512)             # 1024 > 400 -> 512
+
+
+# ------------------------------------------------------------ remove peaks
+def _uf_segments(d, strips):
+    """Independent check: union-find over the 4-neighbour edges of each strip
+    (|d - d'| <= 1, both valid) -> root label per processed pixel."""
+    H, W = d.shape
+    rows = H // strips
+    parent = list(range(H * W))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for y in range(rows * strips):
+        for x in range(W):
+            if d[y, x] == oracle.INVALID:
+                continue
+            for yy, xx in ((y, x + 1), (y + 1, x)):
+                if yy >= rows * strips or xx >= W or yy // rows != y // rows:
+                    continue
+                if d[yy, xx] != oracle.INVALID and abs(float(d[yy, xx]) - float(d[y, x])) <= 1.0:
+                    ra, rb = find(y * W + x), find(yy * W + xx)
+                    parent[max(ra, rb)] = min(ra, rb)
+    return np.array([find(i) for i in range(H * W)]).reshape(H, W), rows
+
+
+def test_remove_peaks_hand_case():
+    """Segment sizes 4, 1, 3, 2, 2 against threshold (int)(18 * (3/18)) = 3;
+    |3.5 - 2.4| = 1.1 splits, |2.4 - 1.5| = 0.9 joins."""
+    I = oracle.INVALID
+    d = np.array([[1, 1.5, 2.4, 9, 9, I],
+                  [1, I, 3.5, 9, I, 5],
+                  [7, 7, I, I, I, 5]], np.float32)
+    out, n = oracle.remove_peaks(d, 1, 3 / 18.0)
+    exp = d.copy()
+    exp[1, 2] = exp[1, 5] = exp[2, 5] = exp[2, 0] = exp[2, 1] = I
+    np.testing.assert_array_equal(out, exp)
+    assert n == 5
+
+
+def test_remove_peaks_join_rule_and_chaining():
+    I = oracle.INVALID
+    # a ramp of unit steps is one segment (the rule links neighbours, not the seed)
+    ramp = np.arange(10, dtype=np.float32)[None, :].repeat(2, 0)
+    assert oracle.remove_peaks(ramp, 1, 0.5)[1] == 0           # one segment of 20 >= 10
+    # exactly 1 joins; the next float above 1 does not
+    a = np.array([[2.0, 3.0]], np.float32)
+    b = np.array([[2.0, np.nextafter(np.float32(3.0), np.float32(4.0))]], np.float32)
+    assert oracle.remove_peaks(a, 1, 1.0)[1] == 0                # threshold 2, size 2
+    assert oracle.remove_peaks(b, 1, 1.0)[1] == 2                # two singletons
+    # invalid pixels split segments and are never counted
+    c = np.array([[1, 1, I, 1, 1, 1]], np.float32)
+    out, n = oracle.remove_peaks(c, 1, 0.5)                       # threshold 3
+    assert n == 2 and (out[0, :2] == I).all() and (out[0, 3:] == 1).all()
+
+
+def test_remove_peaks_strips():
+    """H / strips rows per strip, no segment across strips, the rows past the
+    last strip untouched, threshold from one strip's area."""
+    I = oracle.INVALID
+    d = np.full((5, 4), 3.0, np.float32)
+    d[4, 1] = 50.0                                                # a lone peak in the last row
+    out1, n1 = oracle.remove_peaks(d, 1, 0.1)                     # threshold (int)(20*0.1) = 2
+    assert n1 == 1 and out1[4, 1] == I
+    out2, n2 = oracle.remove_peaks(d, 2, 0.7)                     # 2 strips of 2 rows; threshold 5
+    assert n2 == 0 and out2[4, 1] == 50.0                          # row 4 is outside both strips
+    d3 = np.full((4, 4), 3.0, np.float32)
+    d3[1, :] = I                                                  # strip 0 keeps only row 0: 4 px
+    out3, n3 = oracle.remove_peaks(d3, 2, 0.7)                    # threshold (int)(8*0.7) = 5
+    assert n3 == 4 and (out3[0] == I).all() and (out3[2:] == 3.0).all()
+
+
+@pytest.mark.parametrize("strips,ratio,seed", [(1, 0.02, 0), (3, 0.05, 1), (4, 0.01, 2), (7, 0.1, 3)])
+def test_remove_peaks_against_union_find(strips, ratio, seed):
+    rng = np.random.default_rng(seed)
+    H, W = 37, 29
+    d = np.round(rng.uniform(0, 6, size=(H, W)) * 2).astype(np.float32) / 2
+    d += rng.choice([0.0, 0.25], size=(H, W)).astype(np.float32)
+    d[rng.random((H, W)) < 0.15] = oracle.INVALID
+    out, n = oracle.remove_peaks(d, strips, ratio)
+    root, rows = _uf_segments(d, strips)
+    valid = d != oracle.INVALID
+    processed = np.zeros_like(valid)
+    processed[:rows * strips] = True
+    sizes = np.bincount(root[valid & processed].ravel(), minlength=H * W)
+    thr = int(np.float32(np.float32(W * rows) * np.float32(ratio)))
+    exp = d.copy()
+    kill = valid & processed & (sizes[root] < thr)
+    exp[kill] = oracle.INVALID
+    np.testing.assert_array_equal(out, exp)
+    assert n == int(kill.sum()) and n > 0
