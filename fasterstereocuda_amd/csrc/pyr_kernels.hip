@@ -110,56 +110,123 @@ __global__ __launch_bounds__(256) void cost_init_kernel(PyrLevel a) {
 }
 
 // Aggregation over the per-pixel window (cusgm_ca.cu:1181-1339), one
-// disparity per lane.  See the oracle for the step rules.
+// disparity per lane, SPW = 64 / LR scanlines per wave; see the oracle for the
+// step rules.  A scanline is walked in blocks of K = LR steps.  For the loads
+// and stores lane j stands for pixel (line j / K, step j % K) of a block: it
+// fetches that pixel's whole window (LR cost bytes) and base disparity one
+// block ahead, and the block passes through a per-wave LDS area, from which
+// every step reads its byte and base.  Each step's results are gathered
+// there and leave as one LR-byte store per lane and block.  (Loading one
+// byte per lane and step, and waiting for it, made the scanline walk
+// latency-bound.)  Pixels whose base is invalid are never read by the WTA;
+// they are written as 0.
 template <int LR>
-__device__ __forceinline__ void agg_init_lines(const PyrLevel& a, int dir, int group, int lane) {
-    constexpr int SPW = 64 / LR;               // scanlines per wave
-    const int sub = lane / LR, t = lane % LR;
+struct AggInitLds {
+    uint8_t cost[64 * LR];   // [line][step][t]
+    uint8_t out[64 * LR];
+    int16_t init[64];        // [line][step]
+};
+
+template <int LR>
+__device__ __forceinline__ void agg_init_lines(const PyrLevel& a, int dir, int group, int lane,
+                                               AggInitLds<LR>& S) {
+    constexpr int SPW = 64 / LR, K = LR, NV = LR / 16;
+    const int sub = lane / LR, t = lane % LR;        // recurrence: line, disparity
+    const int lsub = lane / K, lstep = lane % K;     // loads / stores: line, step
     const bool horiz = (dir == 1 || dir == 3);
     const bool fwd = (dir == 0 || dir == 1);
     const int W = a.w, H = a.h;
     const int nlines = horiz ? H : W;
-    int line = group * SPW + sub;
-    const bool active = line < nlines;
-    if (!active) line = nlines - 1;
     const int n = horiz ? W : H;
-    const int64_t pix0 = horiz ? (int64_t)line * W + (fwd ? 0 : W - 1)
-                               : (int64_t)(fwd ? 0 : H - 1) * W + line;
-    const int64_t pstep = horiz ? (fwd ? 1 : -1) : (fwd ? W : -W);
+    // pixel of (line, step); lines past the last and steps past the end are
+    // clamped (computed, never stored)
+    auto pix_of = [&](int line, int step) -> int64_t {
+        line = min(line, nlines - 1);
+        step = min(step, n - 1);
+        const int pos = fwd ? step : n - 1 - step;
+        return horiz ? (int64_t)line * W + pos : (int64_t)pos * W + line;
+    };
+    const int line = min(group * SPW + sub, nlines - 1);
+    const int lline = group * SPW + lsub;
     uint8_t* out = (dir == 0) ? a.path[0] : (dir == 1) ? a.path[1] : (dir == 2) ? a.path[2] : a.path[3];
     const uint32_t p1 = (uint32_t)a.p1;
     const int gbase = sub * LR;
 
-    int64_t p = pix0;
-    uint32_t L = a.cost[p * LR + t];
-    if (active) out[p * LR + t] = (uint8_t)L;
-    uint32_t m = group_min<LR>(L, lane);
-    // A11: vertical directions seed b' from row 0 whichever end they start at
-    int last_b = horiz ? a.init[p] : a.init[line];
-    for (int s = 1; s < n; ++s) {
-        p += pstep;
-        const int b = a.init[p];
-        const bool valid = b != kInvalidShort;
-        const uint32_t c = a.cost[p * LR + t];
-        const int i1 = t + b - 1 - last_b, i2 = t + b + 1 - last_b;
-        const uint32_t v1 = bperm(gbase + min(max(i1, 0), LR - 1), L);
-        const uint32_t v2 = bperm(gbase + min(max(i2, 0), LR - 1), L);
-        const uint32_t bm = (i1 >= 0 && i1 < LR) ? v1 + p1 : 127u;
-        const uint32_t bp = (i2 >= 0 && i2 < LR) ? v2 + p1 : 127u;
-        const int div = max(1, abs(last_b - b + 1));
-        const uint32_t pen2 = (uint32_t)max(a.p1, a.p2 / div);
-        const uint32_t v = min(min(L, bm), min(bp, m + pen2));
-        const uint32_t l = c + (v - m);
-        const uint32_t mn = group_min<LR>(l, lane);
-        if (valid && active) out[p * LR + t] = (uint8_t)l;
-        L = valid ? l : L;
-        m = valid ? mn : 127u;
-        last_b = valid ? b : last_b;
+    static_assert(NV == 1 || NV == 2, "windows of 16 or 32 bytes");
+    uint4 nc0 = make_uint4(0u, 0u, 0u, 0u), nc1 = nc0;   // named, so they stay in registers
+    int nb = 0;
+    auto fetch = [&](int blk) __attribute__((always_inline)) {
+        const int64_t p = pix_of(lline, blk * K + lstep);
+        const uint4* src = reinterpret_cast<const uint4*>(a.cost + p * LR);
+        nc0 = src[0];
+        if constexpr (NV == 2) nc1 = src[1];
+        nb = a.init[p];
+    };
+    auto stage = [&]() __attribute__((always_inline)) {
+        uint4* dst = reinterpret_cast<uint4*>(S.cost + lane * LR);
+        dst[0] = nc0;
+        if constexpr (NV == 2) dst[1] = nc1;
+        S.init[lane] = (int16_t)nb;
+    };
+
+    uint32_t L = 0, m = 0;
+    int last_b = 0;
+    const int nblk = (n + K - 1) / K;
+    fetch(0);
+    stage();
+    if (nblk > 1) fetch(1);
+    for (int blk = 0; blk < nblk; ++blk) {
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            const uint32_t c = S.cost[(sub * K + s) * LR + t];
+            const int b = S.init[sub * K + s];
+            uint32_t o;
+            if (s == 0 && blk == 0) {
+                L = c;
+                m = group_min<LR>(L, lane);
+                // A11: vertical directions seed b' from row 0 whichever end they start at
+                last_b = horiz ? b : a.init[line];
+                o = L;
+            } else {
+                const bool valid = b != kInvalidShort;
+                const int i1 = t + b - 1 - last_b, i2 = t + b + 1 - last_b;
+                const uint32_t v1 = bperm(gbase + min(max(i1, 0), LR - 1), L);
+                const uint32_t v2 = bperm(gbase + min(max(i2, 0), LR - 1), L);
+                const uint32_t bm = (i1 >= 0 && i1 < LR) ? v1 + p1 : 127u;
+                const uint32_t bp = (i2 >= 0 && i2 < LR) ? v2 + p1 : 127u;
+                const int div = max(1, abs(last_b - b + 1));
+                const uint32_t pen2 = (uint32_t)max(a.p1, a.p2 / div);
+                const uint32_t v = min(min(L, bm), min(bp, m + pen2));
+                const uint32_t l = c + (v - m);
+                const uint32_t mn = group_min<LR>(l, lane);
+                o = valid ? l : 0u;
+                L = valid ? l : L;
+                m = valid ? mn : 127u;
+                last_b = valid ? b : last_b;
+            }
+            S.out[(sub * K + s) * LR + t] = (uint8_t)o;
+        }
+        __builtin_amdgcn_wave_barrier();
+        {   // this block's results, one window per lane
+            const int step = blk * K + lstep;
+            if (lline < nlines && step < n) {
+                uint4* dst = reinterpret_cast<uint4*>(out + pix_of(lline, step) * LR);
+                const uint4* src = reinterpret_cast<const uint4*>(S.out + lane * LR);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) dst[q] = src[q];
+            }
+        }
+        if (blk + 1 < nblk) {
+            stage();
+            if (blk + 2 < nblk) fetch(blk + 2);
+        }
     }
 }
 
 template <int LR>
 __global__ __launch_bounds__(64) void agg_init_kernel(PyrLevel a) {
+    __shared__ __align__(16) AggInitLds<LR> lds;
     constexpr int SPW = 64 / LR;
     const int nh = (a.h + SPW - 1) / SPW, nv = (a.w + SPW - 1) / SPW;
     const int item = blockIdx.x;
@@ -168,7 +235,7 @@ __global__ __launch_bounds__(64) void agg_init_kernel(PyrLevel a) {
     else if (item < 2 * nh) { dir = 3; group = item - nh; }
     else if (item < 2 * nh + nv) { dir = 0; group = item - 2 * nh; }
     else { dir = 2; group = item - 2 * nh - nv; }
-    agg_init_lines<LR>(a, dir, group, threadIdx.x);
+    agg_init_lines<LR>(a, dir, group, threadIdx.x, lds);
 }
 
 // WTA with initial disparity (cusgm_ca.cu:3425-3512), one lane per pixel
