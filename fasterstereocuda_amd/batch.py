@@ -39,15 +39,16 @@ class SgmBatch:
                                  p["disp_left"].data_ptr(), None if dr is None else dr.data_ptr())
             self._keep.append(p)
         opt = option.to_c()
-        self._b = lib().sgm_batch_create(arr, len(pairs), ctypes.byref(opt))
+        self._lib = lib()    # kept for __del__, which may run after module teardown
+        self._b = self._lib.sgm_batch_create(arr, len(pairs), ctypes.byref(opt))
         if not self._b:
             raise RuntimeError(f"sgm_batch_create failed: {last_error()}")
         self.n = len(pairs)
 
     def __del__(self):
-        b = getattr(self, "_b", None)
-        if b:
-            lib().sgm_batch_destroy(b)
+        b, L = getattr(self, "_b", None), getattr(self, "_lib", None)
+        if b and L is not None:
+            L.sgm_batch_destroy(b)
             self._b = None
 
     @property

@@ -24,8 +24,9 @@ class SgmPyramid:
     def __init__(self, width: int, height: int, min_disp: int, max_disp: int, num_layers: int,
                  level_range: int, option: CuSGMOption):
         opt = option.to_c()
-        self._p = lib().sgm_pyramid_create(width, height, min_disp, max_disp, num_layers,
-                                           level_range, ctypes.byref(opt))
+        self._lib = lib()    # kept for __del__, which may run after module teardown
+        self._p = self._lib.sgm_pyramid_create(width, height, min_disp, max_disp, num_layers,
+                                               level_range, ctypes.byref(opt))
         if not self._p:
             raise RuntimeError(f"sgm_pyramid_create failed: {last_error()}")
         self.width, self.height = width, height
@@ -38,17 +39,18 @@ class SgmPyramid:
         (HierSgmCudaImpl::Init2); match2 then returns depth."""
         self = cls.__new__(cls)
         opt, c = option.to_c(), cam.to_c()
-        self._p = lib().sgm_pyramid_create2(width, height, min_depth, max_depth, num_layers,
-                                            level_range, ctypes.byref(opt), ctypes.byref(c))
+        self._lib = lib()
+        self._p = self._lib.sgm_pyramid_create2(width, height, min_depth, max_depth, num_layers,
+                                                level_range, ctypes.byref(opt), ctypes.byref(c))
         if not self._p:
             raise RuntimeError(f"sgm_pyramid_create2 failed: {last_error()}")
         self.width, self.height = width, height
         return self
 
     def __del__(self):
-        p = getattr(self, "_p", None)
-        if p:
-            lib().sgm_pyramid_destroy(p)
+        p, L = getattr(self, "_p", None), getattr(self, "_lib", None)
+        if p and L is not None:
+            L.sgm_pyramid_destroy(p)
             self._p = None
 
     @property

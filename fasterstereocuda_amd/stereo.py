@@ -114,16 +114,17 @@ class StereoCuda:
     """Single-level SGM matcher (StereoCuda.h)."""
 
     def __init__(self):
-        self._h = lib().sgm_create()
+        self._lib = lib()    # kept for __del__, which may run after module teardown
+        self._h = self._lib.sgm_create()
         if not self._h:
             raise MemoryError("sgm_create failed")
         self.width = self.height = 0
         self.last_error = ""
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            lib().sgm_destroy(h)
+        h, L = getattr(self, "_h", None), getattr(self, "_lib", None)
+        if h and L is not None:
+            L.sgm_destroy(h)
             self._h = None
 
     def _ok(self, rc: int) -> bool:
