@@ -125,9 +125,10 @@ struct AggInitLds {
     uint8_t cost[64 * LR];   // [line][step][t]
     uint8_t out[64 * LR];
     int16_t init[64];        // [line][step]
+    uint32_t pen2[256];      // max(P1, P2 / k), k = max(1, index)
 };
 
-template <int LR>
+template <int LR, bool PEN2_TABLE>
 __device__ __forceinline__ void agg_init_lines(const PyrLevel& a, int dir, int group, int lane,
                                                AggInitLds<LR>& S) {
     constexpr int SPW = 64 / LR, K = LR, NV = LR / 16;
@@ -169,6 +170,15 @@ __device__ __forceinline__ void agg_init_lines(const PyrLevel& a, int dir, int g
         S.init[lane] = (int16_t)nb;
     };
 
+    // P2 / max(1, |b' - b + 1|) by table (PEN2_TABLE, for P2 <= 254): every
+    // divisor from 255 on gives P2 / k = 0, the same as entry 255, so the
+    // table is exact.  A larger P2 divides (an integer division costs ~15
+    // VALU per step).
+    if constexpr (PEN2_TABLE) {
+#pragma unroll
+        for (int k = lane; k < 256; k += 64) S.pen2[k] = (uint32_t)max(a.p1, a.p2 / max(1, k));
+    }
+
     uint32_t L = 0, m = 0;
     int last_b = 0;
     const int nblk = (n + K - 1) / K;
@@ -196,7 +206,9 @@ __device__ __forceinline__ void agg_init_lines(const PyrLevel& a, int dir, int g
                 const uint32_t bm = (i1 >= 0 && i1 < LR) ? v1 + p1 : 127u;
                 const uint32_t bp = (i2 >= 0 && i2 < LR) ? v2 + p1 : 127u;
                 const int div = max(1, abs(last_b - b + 1));
-                const uint32_t pen2 = (uint32_t)max(a.p1, a.p2 / div);
+                uint32_t pen2;
+                if constexpr (PEN2_TABLE) pen2 = S.pen2[min(div, 255)];
+                else pen2 = (uint32_t)max(a.p1, a.p2 / div);
                 const uint32_t v = min(min(L, bm), min(bp, m + pen2));
                 const uint32_t l = c + (v - m);
                 const uint32_t mn = group_min<LR>(l, lane);
@@ -235,7 +247,8 @@ __global__ __launch_bounds__(64) void agg_init_kernel(PyrLevel a) {
     else if (item < 2 * nh) { dir = 3; group = item - nh; }
     else if (item < 2 * nh + nv) { dir = 0; group = item - 2 * nh; }
     else { dir = 2; group = item - 2 * nh - nv; }
-    agg_init_lines<LR>(a, dir, group, threadIdx.x, lds);
+    if (a.p2 <= 254) agg_init_lines<LR, true>(a, dir, group, threadIdx.x, lds);
+    else agg_init_lines<LR, false>(a, dir, group, threadIdx.x, lds);
 }
 
 // WTA with initial disparity (cusgm_ca.cu:3425-3512), one lane per pixel

@@ -50,6 +50,25 @@ def test_window_stages_match_oracle(cuda, lr, cs):
         np.testing.assert_array_equal(_bits(d.cpu().numpy()), _bits(oracle.wta_init(ref_paths, init, uniq)))
 
 
+@pytest.mark.parametrize("lr", [16, 32])
+@pytest.mark.parametrize("p1,p2", [(10, 254), (20, 400), (0, 60000)])
+def test_window_aggregation_penalty_forms(cuda, lr, p1, p2):
+    """P2 / max(1, |b' - b + 1|) comes from a table up to P2 = 254 and from a
+    division above; both against the oracle, with base jumps of every size."""
+    print(f"[window aggregation] lr={lr} p1={p1} p2={p2}", flush=True)
+    rng = np.random.default_rng(lr * 7 + p2)
+    h, w = 41, 203
+    init = rng.integers(-40, 300, size=(h, w)).astype(np.int16)   # large jumps
+    init[:, ::3] = init[:, 1::3][:, : init[:, ::3].shape[1]]          # and runs of equal bases
+    init[rng.random((h, w)) < 0.1] = oracle.INVALID_SHORT
+    cost = rng.integers(0, 64, size=(h, w, lr), dtype=np.uint8)
+    paths = stages.aggregate_init(torch.from_numpy(cost).to(cuda), torch.from_numpy(init).to(cuda),
+                                  p1, p2)
+    for r in range(4):
+        np.testing.assert_array_equal(paths[r].cpu().numpy(),
+                                      oracle.aggregate_init(cost, init, r, p1, p2), err_msg=f"dir {r}")
+
+
 @pytest.mark.parametrize("w,h", [(130, 57), (131, 58), (64, 33)])
 def test_zoom_and_init_propagation(cuda, w, h):
     rng = np.random.default_rng(w * h)
