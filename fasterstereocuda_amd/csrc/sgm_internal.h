@@ -153,7 +153,6 @@ bool choose_wide(bool eligible, int n_wh);
 hipError_t launch_wta(const StagePlan& p, hipStream_t s);
 hipError_t launch_post(const StagePlan& p, hipStream_t s);
 hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s);
-hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t s);
 // peaks.hip: segments of fewer than (int)((float)(W r) ratio) pixels of each
 // r = H / strips row strip become invalid (in place); scratch = 2 W H ints
 hipError_t launch_remove_peaks(float* disp, int* scratch, int W, int H, int strips, float ratio,

@@ -631,11 +631,6 @@ __global__ __launch_bounds__(256) void right_invalidate_kernel(const SgmPost* __
     if (fabsf(dl - dr) > P.lr_thr) gst(P.out_r + (int64_t)y * P.W + xr, kInvalidDisp);
 }
 
-__global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ dst, size_t n, float v) {
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
-        dst[i] = v;
-}
-
 // Depth from disparity (cusgm_util.cu:7-33): f1 = baseline * focus and
 // f2 = x0_right - x0_left, both float64; the division is IEEE float64 and the
 // result is rounded once to float32.  Invalid pixels stay invalid.
@@ -655,13 +650,6 @@ hipError_t launch_depth(const float* disp, float* depth, size_t n, double f1, do
     if (n == 0) return hipSuccess;
     const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
     hipLaunchKernelGGL(depth_kernel, dim3(blocks), dim3(256), 0, s, disp, depth, n, f1, f2);
-    return hipGetLastError();
-}
-
-hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
-    hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, s, dst, n, value);
     return hipGetLastError();
 }
 
