@@ -112,6 +112,10 @@ bool pyramid_args_ok(int32_t width, int32_t height, int32_t num_layers, int32_t 
                          "pyramid: post filters, morphology and adaptive P2 are not built");
         return false;
     }
+    if (opt->do_remove_peaks && (int64_t)width * height > INT32_MAX) {
+        set_global_error(SGM_E_INVALID_ARG, "pyramid: peak removal needs width * height < 2^31");
+        return false;
+    }
     if (opt->do_remove_peaks &&
         !(opt->peaks_ratio_threshold >= 0.0f && opt->peaks_ratio_threshold <= 1.0f)) {
         set_global_error(SGM_E_INVALID_ARG, "pyramid: peaks_ratio_threshold must be in [0, 1]");

@@ -395,6 +395,8 @@ int sgm_init(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity
     if (std::max(0, min_disparity) >= width)
         return fail(&h->err, SGM_E_INVALID_ARG, "min_disparity leaves no column to match");
     if (int rc = check_option(opt, &h->err, true)) return rc;
+    if (opt->do_remove_peaks && (int64_t)width * height > INT32_MAX)
+        return fail(&h->err, SGM_E_INVALID_ARG, "peak removal needs width * height < 2^31");
     sgm_release(h);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)

@@ -105,6 +105,15 @@ def test_init_validates_before_device(args, code, needle):
     lib.sgm_destroy(h)
 
 
+def test_peak_removal_size_limit():
+    lib = _lib.lib()
+    h = lib.sgm_create()
+    opt = CuSGMOption.benchmark(do_remove_peaks=True, peaks_ratio_threshold=0.001).to_c()
+    assert lib.sgm_init(h, 50000, 50000, 0, 64, ctypes.byref(opt), 0) == _lib.SGM_E_INVALID_ARG
+    assert "2^31" in _lib.last_error(h)
+    lib.sgm_destroy(h)
+
+
 @pytest.mark.parametrize("fields,code", [
     ({"do_remove_peaks": True, "peaks_ratio_threshold": 2.0}, _lib.SGM_E_INVALID_ARG),
     ({"do_remove_peaks": True, "peaks_ratio_threshold": -0.1}, _lib.SGM_E_INVALID_ARG),
