@@ -257,8 +257,11 @@ def cpu_model() -> str:
 
 
 def host_match_rates(opt, reps: int = 20):
-    """ms per StereoCuda.Match on host numpy buffers (PCIe copies included)."""
+    """ms per StereoCuda.Match on host buffers, PCIe copies included: page-locked
+    buffers (the reference console allocates them with MallocPageLockedPtr) and,
+    beside them, ordinary pageable numpy arrays."""
     import numpy as np
+    import torch
     from fasterstereocuda_amd import scenes
     from fasterstereocuda_amd.stereo import StereoCuda
     out = {}
@@ -267,15 +270,24 @@ def host_match_rates(opt, reps: int = 20):
         H, W = pair.left.shape
         st = StereoCuda()
         assert st.Init(W, H, 0, 64, opt), st.last_error
-        dl = np.empty((H, W), np.float32)
-        dr = np.empty((H, W), np.float32)
-        st.Match(pair.left, pair.right, dl, dr)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            st.Match(pair.left, pair.right, dl, dr)
-        ms = (time.perf_counter() - t0) * 1e3 / reps
+
+        def rate(left, right, dl, dr):
+            st.Match(left, right, dl, dr)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                st.Match(left, right, dl, dr)
+            return (time.perf_counter() - t0) * 1e3 / reps
+
+        pinned = [torch.from_numpy(pair.left).pin_memory().numpy(),
+                  torch.from_numpy(pair.right).pin_memory().numpy(),
+                  torch.empty((H, W), dtype=torch.float32).pin_memory().numpy(),
+                  torch.empty((H, W), dtype=torch.float32).pin_memory().numpy()]
+        ms = rate(*pinned)
+        ms_pageable = rate(pair.left, pair.right, np.empty((H, W), np.float32),
+                           np.empty((H, W), np.float32))
         st.Release()
-        out[name] = {"ms_per_pair": ms, "W*H*D/s": W * H * 64 / (ms * 1e-3)}
+        out[name] = {"ms_per_pair": ms, "W*H*D/s": W * H * 64 / (ms * 1e-3),
+                     "ms_per_pair_pageable": ms_pageable}
     return out
 
 

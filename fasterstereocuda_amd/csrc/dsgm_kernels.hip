@@ -171,7 +171,29 @@ __device__ __forceinline__ int row_argmin(const float (&L)[DPD], float m, int l)
 }
 
 // ------------------------------------------------------------- forward ---
-template <int D>
+// S(p) = ((L_0 + L_1) + L_2) + L_3 and its soft-argmin, for one pixel per
+// 16-lane row: s receives S, the row's lane 0 the disparity.
+template <int DPD>
+__device__ __forceinline__ float soft_argmin(const float (&s)[DPD], int l, float invT) {
+    const float mn = row_minf(lane_min(s));
+    float z = 0.f, acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < DPD; ++k) {
+        const float e = expf(-(s[k] - mn) * invT);
+        z += e;
+        acc += (float)(l * DPD + k) * e;
+    }
+    z = row_sumf(z);
+    acc = row_sumf(acc);
+    return acc / z;
+}
+
+// One direction over a scanline group.  FUSE (direction 2, bottom->top,
+// launched after the other three): instead of storing its volume, every
+// step forms S from the other three volumes and its own L, in the fixed
+// order ((L_0 + L_1) + L_2) + L_3, and writes S and the soft-argmin -- the
+// separate sum pass would read all four volumes back.
+template <int D, bool FUSE>
 __device__ __forceinline__ void fwd_lines(const DsgmArgs& A, int b, int path, int group, int lane) {
     constexpr int DPD = D / 16;
     const int sub = lane >> 4, l = lane & 15;
@@ -181,14 +203,39 @@ __device__ __forceinline__ void fwd_lines(const DsgmArgs& A, int b, int path, in
     const float* p1 = A.p1 + (int64_t)b * img;
     const float* p2 = A.p2 + (int64_t)b * img;
     float* out = A.paths + ((int64_t)b * 4 + path) * img * D + l * DPD;
+    const float* vol = A.paths + (int64_t)b * 4 * img * D + l * DPD;   // direction r at r * img * D
+    float* S = A.S + (int64_t)b * img * D + l * DPD;
+    float* disp = A.disp + (int64_t)b * img;
     float* ck = A.ckpt + (int64_t)b * dsgm_ckpt_floats_per_image(A.H, A.W, D) + g.ckpt0 + l * DPD;
+    const float invT = A.inv_temperature;
+
+    // the other three directions at pixel q (FUSE)
+    float o0[DPD], o1[DPD], o3[DPD];
+    auto load_others = [&](int64_t q) __attribute__((always_inline)) {
+        loadf<DPD>(vol + q * D, o0);
+        loadf<DPD>(vol + (img + q) * D, o1);
+        loadf<DPD>(vol + (3 * img + q) * D, o3);
+    };
+    auto emit = [&](int64_t q, const float (&L)[DPD]) __attribute__((always_inline)) {
+        if constexpr (FUSE) {
+            float sv[DPD];
+#pragma unroll
+            for (int k = 0; k < DPD; ++k) sv[k] = ((o0[k] + o1[k]) + L[k]) + o3[k];
+            const float d = soft_argmin<DPD>(sv, l, invT);
+            if (g.active) {
+                storef<DPD>(S + q * D, sv);
+                if (l == 0) disp[q] = d;
+            }
+        } else {
+            if (g.active) storef<DPD>(out + q * D, L);
+        }
+    };
 
     float L[DPD];
     loadf<DPD>(cost + g.pix0 * D, L);
-    if (g.active) {
-        storef<DPD>(out + g.pix0 * D, L);
-        storef<DPD>(ck, L);
-    }
+    if constexpr (FUSE) load_others(g.pix0);
+    emit(g.pix0, L);
+    if (g.active) storef<DPD>(ck, L);
     float m = row_minf(lane_min(L));
     int64_t pix = g.pix0;
     float c[DPD];
@@ -200,68 +247,38 @@ __device__ __forceinline__ void fwd_lines(const DsgmArgs& A, int b, int path, in
         for (int k = 0; k < DPD; ++k) cur[k] = c[k];
         const int64_t nxt = (s + 1 < g.n) ? pix + g.pstep : pix;
         loadf<DPD>(cost + nxt * D, c);
+        if constexpr (FUSE) load_others(pix);
         (void)fwd_step<DPD>(L, cur, m, loads(p1 + pix), loads(p2 + pix));
         m = row_minf(lane_min(L));
-        if (g.active) {
-            storef<DPD>(out + pix * D, L);
-            if ((s & (kSeg - 1)) == 0) storef<DPD>(ck + (int64_t)(s / kSeg) * D, L);
-        }
+        emit(pix, L);
+        if (g.active && (s & (kSeg - 1)) == 0) storef<DPD>(ck + (int64_t)(s / kSeg) * D, L);
     }
 }
 
+// Items of a launch over directions 1, 3, 0 (LAST = false) or over
+// direction 2 alone (LAST = true), image by image.
+template <bool LAST>
 __device__ __forceinline__ void item_of(const DsgmArgs& A, int item, int* b, int* path, int* group) {
     const int nh = (A.H + 3) / 4, nv = (A.W + 3) / 4;
-    const int per = 2 * nh + 2 * nv;
+    if constexpr (LAST) {
+        *b = item / nv;
+        *path = 2;
+        *group = item - *b * nv;
+        return;
+    }
+    const int per = 2 * nh + nv;
     *b = item / per;
     const int local = item - *b * per;
     if (local < nh) { *path = 1; *group = local; }
     else if (local < 2 * nh) { *path = 3; *group = local - nh; }
-    else if (local < 2 * nh + nv) { *path = 0; *group = local - 2 * nh; }
-    else { *path = 2; *group = local - 2 * nh - nv; }
+    else { *path = 0; *group = local - 2 * nh; }
 }
 
-template <int D>
+template <int D, bool FUSE>
 __global__ __launch_bounds__(64) void dsgm_forward_kernel(DsgmArgs A) {
     int b, path, group;
-    item_of(A, blockIdx.x, &b, &path, &group);
-    fwd_lines<D>(A, b, path, group, threadIdx.x);
-}
-
-// S = sum of the four directions, soft-argmin disparity.  16 lanes / pixel.
-template <int D>
-__global__ __launch_bounds__(256) void dsgm_softargmin_kernel(DsgmArgs A) {
-    constexpr int DPD = D / 16;
-    const int64_t img = (int64_t)A.H * A.W;
-    const int64_t p = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-    const int l = threadIdx.x & 15;
-    const bool ok = p < (int64_t)A.B * img;
-    const int64_t pc = ok ? p : 0;
-    const int64_t b = pc / img, q = pc - b * img;
-    float s[DPD];
-#pragma unroll
-    for (int k = 0; k < DPD; ++k) s[k] = 0.f;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        float v[DPD];
-        loadf<DPD>(A.paths + ((b * 4 + r) * img + q) * D + l * DPD, v);
-#pragma unroll
-        for (int k = 0; k < DPD; ++k) s[k] = s[k] + v[k];
-    }
-    const float mn = row_minf(lane_min(s));
-    float z = 0.f, acc = 0.f;
-    const float invT = A.inv_temperature;
-#pragma unroll
-    for (int k = 0; k < DPD; ++k) {
-        const float e = expf(-(s[k] - mn) * invT);
-        z += e;
-        acc += (float)(l * DPD + k) * e;
-    }
-    z = row_sumf(z);
-    acc = row_sumf(acc);
-    if (ok) {
-        storef<DPD>(A.S + pc * D + l * DPD, s);
-        if (l == 0) A.disp[pc] = acc / z;
-    }
+    item_of<FUSE>(A, blockIdx.x, &b, &path, &group);
+    fwd_lines<D, FUSE>(A, b, path, group, threadIdx.x);
 }
 
 // ------------------------------------------------------------ backward ---
@@ -295,7 +312,11 @@ __global__ __launch_bounds__(256) void dsgm_grad_s_kernel(DsgmArgs A) {
     if (ok) storef<DPD>(A.grad_s + pc * D + l * DPD, g);
 }
 
-template <int D>
+// One direction's reverse recurrence over a scanline group.  FUSE
+// (direction 2, launched after the other three): instead of storing its
+// gradients, every pixel adds them to the other three directions' in the
+// fixed order ((g_0 + g_1) + g_2) + g_3 and writes dC, dP1 and dP2.
+template <int D, bool FUSE>
 __device__ __forceinline__ void bwd_lines(const DsgmArgs& A, int b, int path, int group, int lane) {
     constexpr int DPD = D / 16;
     const int sub = lane >> 4, l = lane & 15;
@@ -309,7 +330,33 @@ __device__ __forceinline__ void bwd_lines(const DsgmArgs& A, int b, int path, in
     float* gc = A.grad_paths + ((int64_t)b * 4 + path) * img * D + l * DPD;
     float* gp1 = A.grad_p1_paths + ((int64_t)b * 4 + path) * img;
     float* gp2 = A.grad_p2_paths + ((int64_t)b * 4 + path) * img;
+    const float* gv = A.grad_paths + (int64_t)b * 4 * img * D + l * DPD;   // direction r at r * img * D
+    const float* a1v = A.grad_p1_paths + (int64_t)b * 4 * img;
+    const float* a2v = A.grad_p2_paths + (int64_t)b * 4 * img;
+    float* out_c = A.grad_cost + (int64_t)b * img * D + l * DPD;
+    float* out_p1 = A.grad_p1 + (int64_t)b * img;
+    float* out_p2 = A.grad_p2 + (int64_t)b * img;
     const int n = g.n;
+    // this direction's gradients at pixel q: stored, or summed with the others
+    auto emit = [&](int64_t q, const float (&G)[DPD], float d1, float d2) __attribute__((always_inline)) {
+        if (!g.active) return;
+        if constexpr (FUSE) {
+            float o0[DPD], o1[DPD], o3[DPD], sv[DPD];
+            loadf<DPD>(gv + q * D, o0);
+            loadf<DPD>(gv + (img + q) * D, o1);
+            loadf<DPD>(gv + (3 * img + q) * D, o3);
+#pragma unroll
+            for (int k = 0; k < DPD; ++k) sv[k] = ((o0[k] + o1[k]) + G[k]) + o3[k];
+            storef<DPD>(out_c + q * D, sv);
+            if (l == 0) {
+                out_p1[q] = ((a1v[q] + a1v[img + q]) + d1) + a1v[3 * img + q];
+                out_p2[q] = ((a2v[q] + a2v[img + q]) + d2) + a2v[3 * img + q];
+            }
+        } else {
+            storef<DPD>(gc + q * D, G);
+            if (l == 0) { gp1[q] = d1; gp2[q] = d2; }
+        }
+    };
 
     float G[DPD];
     loadf<DPD>(gs + (g.pix0 + (int64_t)(n - 1) * g.pstep) * D, G);
@@ -360,10 +407,7 @@ __device__ __forceinline__ void bwd_lines(const DsgmArgs& A, int b, int path, in
                 all = row_sumf(all);
                 s3 = row_sumf(s3);
                 s12 = row_sumf(s12);
-                if (g.active) {
-                    storef<DPD>(gc + pix * D, G);
-                    if (l == 0) { gp1[pix] = s12; gp2[pix] = s3; }
-                }
+                emit(pix, G, s12, s3);
                 const float a1_next = dppf<kShl1>(0.f, A1[0]);        // lane+1, d = lane end + 1
                 const float a2_prev = dppf<kShr1>(0.f, A2[DPD - 1]);  // lane-1, d = lane start - 1
                 const int ks = (int)((kst[t >> 2] >> (8 * (t & 3))) & 0xFFu);
@@ -385,37 +429,14 @@ __device__ __forceinline__ void bwd_lines(const DsgmArgs& A, int b, int path, in
             }
         }
     }
-    if (g.active) {
-        storef<DPD>(gc + g.pix0 * D, G);
-        if (l == 0) { gp1[g.pix0] = 0.f; gp2[g.pix0] = 0.f; }   // no penalty at the first pixel
-    }
+    emit(g.pix0, G, 0.f, 0.f);   // no penalty at the first pixel
 }
 
-template <int D>
+template <int D, bool FUSE>
 __global__ __launch_bounds__(64) void dsgm_backward_kernel(DsgmArgs A) {
     int b, path, group;
-    item_of(A, blockIdx.x, &b, &path, &group);
-    bwd_lines<D>(A, b, path, group, threadIdx.x);
-}
-
-// dC = sum of the four directions (fixed order), dP1 / dP2 likewise
-__global__ __launch_bounds__(256) void dsgm_sum_kernel(DsgmArgs A) {
-    const int64_t img = (int64_t)A.H * A.W;
-    const int64_t cells = img * A.D;
-    const int64_t total = (int64_t)A.B * cells;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * 256) {
-        const int64_t b = i / cells, q = i - b * cells;
-        const float* g = A.grad_paths + b * 4 * cells + q;
-        A.grad_cost[i] = ((g[0] + g[cells]) + g[2 * cells]) + g[3 * cells];
-        if (q < img) {
-            const int64_t o = b * 4 * img + q;
-            const float* a = A.grad_p1_paths + o;
-            const float* c = A.grad_p2_paths + o;
-            A.grad_p1[b * img + q] = ((a[0] + a[img]) + a[2 * img]) + a[3 * img];
-            A.grad_p2[b * img + q] = ((c[0] + c[img]) + c[2 * img]) + c[3 * img];
-        }
-    }
+    item_of<FUSE>(A, blockIdx.x, &b, &path, &group);
+    bwd_lines<D, FUSE>(A, b, path, group, threadIdx.x);
 }
 
 }  // namespace
@@ -429,19 +450,28 @@ __global__ __launch_bounds__(256) void dsgm_sum_kernel(DsgmArgs A) {
             default: return hipErrorInvalidValue;                                       \
         }                                                                               \
     } while (0)
+#define FSGM_DSGM_DISPATCH2(KER, FUSE, grid, block, A, s)                                \
+    do {                                                                                \
+        switch ((A).D) {                                                                \
+            case 64: hipLaunchKernelGGL((KER<64, FUSE>), grid, block, 0, s, A); break;  \
+            case 128: hipLaunchKernelGGL((KER<128, FUSE>), grid, block, 0, s, A); break; \
+            case 256: hipLaunchKernelGGL((KER<256, FUSE>), grid, block, 0, s, A); break; \
+            default: return hipErrorInvalidValue;                                       \
+        }                                                                               \
+    } while (0)
 
-static int dsgm_line_items(const DsgmArgs& A) {
-    return A.B * (2 * ((A.H + 3) / 4) + 2 * ((A.W + 3) / 4));
-}
+// scanline groups of directions 1, 3 and 0, and of direction 2
+static int dsgm_items3(const DsgmArgs& A) { return A.B * (2 * ((A.H + 3) / 4) + (A.W + 3) / 4); }
+static int dsgm_items1(const DsgmArgs& A) { return A.B * ((A.W + 3) / 4); }
 
+// directions 1, 3, 0 store their volumes; direction 2 then sums the four and
+// takes the soft-argmin
 hipError_t dsgm_launch_forward(const DsgmArgs& A, hipStream_t s) {
-    const dim3 g1(dsgm_line_items(A)), b1(64);
-    FSGM_DSGM_DISPATCH(dsgm_forward_kernel, g1, b1, A, s);
+    const dim3 b1(64);
+    FSGM_DSGM_DISPATCH2(dsgm_forward_kernel, false, dim3(dsgm_items3(A)), b1, A, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int64_t npix = (int64_t)A.B * A.H * A.W;
-    const dim3 g2((unsigned)((npix + 15) / 16)), b2(256);
-    FSGM_DSGM_DISPATCH(dsgm_softargmin_kernel, g2, b2, A, s);
+    FSGM_DSGM_DISPATCH2(dsgm_forward_kernel, true, dim3(dsgm_items1(A)), b1, A, s);
     return hipGetLastError();
 }
 
@@ -451,13 +481,10 @@ hipError_t dsgm_launch_backward(const DsgmArgs& A, hipStream_t s) {
     FSGM_DSGM_DISPATCH(dsgm_grad_s_kernel, g0, b0, A, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const dim3 g1(dsgm_line_items(A)), b1(64);
-    FSGM_DSGM_DISPATCH(dsgm_backward_kernel, g1, b1, A, s);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const int64_t cells = npix * A.D;
-    const unsigned blocks = (unsigned)std::min<int64_t>((cells + 255) / 256, 1 << 16);
-    hipLaunchKernelGGL(dsgm_sum_kernel, dim3(blocks), dim3(256), 0, s, A);
+    const dim3 b1(64);
+    FSGM_DSGM_DISPATCH2(dsgm_backward_kernel, false, dim3(dsgm_items3(A)), b1, A, s);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    FSGM_DSGM_DISPATCH2(dsgm_backward_kernel, true, dim3(dsgm_items1(A)), b1, A, s);
     return hipGetLastError();
 }
 
