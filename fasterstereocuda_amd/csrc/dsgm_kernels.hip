@@ -13,8 +13,9 @@
 // Subgradients: the first min candidate in the order [L'(d), L'(d-1)+P1,
 // L'(d+1)+P1, m'+P2] wins; m' = L'(k*) with k* the smallest argmin.
 //
-// Memory: the forward keeps the four direction volumes only until S is
-// formed; for the backward it stores L every 32 steps of every scanline
+// Memory: the forward stores the volumes of directions 1, 3 and 0; direction
+// 2 runs last and forms S from them and its own L at every pixel.  For the
+// backward the forward stores L every 32 steps of every scanline
 // (checkpoints).  The backward walks each scanline's 32-step segments from
 // the last to the first: it recomputes the segment from its checkpoint,
 // keeping the winner codes (2 bits per disparity) and k* of every step in
@@ -22,7 +23,8 @@
 //   G(s-1, d) = gS(s-1, d) + G(s, d)[w(s,d)=0] + G(s, d+1)[w(s,d+1)=1]
 //             + G(s, d-1)[w(s,d-1)=2] + [d = k*(s-1)] (sum_e G(s,e)[w(s,e)=3] - sum_e G(s,e))
 // and emits dC_r(s, d) = G(s, d), dP1_r(s) = sum_d G(s,d)[w in {1,2}],
-// dP2_r(s) = sum_d G(s,d)[w = 3]; a last kernel sums the four directions.
+// dP2_r(s) = sum_d G(s,d)[w = 3].  Directions 1, 3, 0 store theirs;
+// direction 2 runs last and adds its own to them at every pixel.
 //
 // Layout per wave as in the integer aggregation: four 16-lane DPP rows, one
 // scanline per row, D/16 disparities per lane.
