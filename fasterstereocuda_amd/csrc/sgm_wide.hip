@@ -6,8 +6,9 @@
 // WTA cusgm_ca.cu:3606-3809), so every volume and disparity is bit-identical;
 // only where the cost comes from and how lanes are laid out differ:
 //
-//   * one scanline = D/DPL lanes ("a line"), DPL disparities per lane (8 or
-//     16) held as packed u16 pairs; a wave covers 64 DPL / D scanlines.  Neighbour terms
+//   * one scanline = D/DPL lanes ("a line"), DPL disparities per lane (16
+//     unless FSGM_DPL_* lowers it) held as packed u16 pairs; a wave covers
+//     64 DPL / D scanlines.  Neighbour terms
 //     d-1 / d+1 across lanes come from row_shr:1 / row_shl:1 DPP moves, the
 //     line minimum from quad_perm / mirror DPP reductions.
 //   * horizontal directions keep each lane's 16 partner census words in a
@@ -17,9 +18,11 @@
 //     over a 16-step unrolled block.  A direction whose window would slide the
 //     other way runs with the disparity order reversed inside the lanes (the
 //     recurrence is symmetric in d); only the byte order of its stores changes.
-//   * vertical directions stage, per step, the wave's own census words and
-//     the D + lines - 1 partner words in a two-slot LDS ring private to the
-//     (single-wave) workgroup; each lane reads its 16 partners from there.
+//   * vertical directions load the census words of the next PF steps into
+//     registers (PF = FSGM_PF_V or FSGM_PF_F) and stage, per step, the wave's
+//     own words and the D + lines - 1 partner words into one of two LDS
+//     slots private to the (single-wave) workgroup; each lane reads its DPL
+//     partners from there.
 //   * cost(x, t) = popcount(own(x) ^ partner(x, t)): 2 v_xor + 2 v_bcnt.
 #include <algorithm>
 
@@ -419,8 +422,9 @@ __device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int la
         if (q == ((i - 1) & 3)) { kb = best; kboth = (c1 << 16) | c3; }
     };
 
-    // rings (two steps ahead) of census words and, with FUSE, of the other
-    // three directions' bytes; the loop body covers 4 steps (one WTA block)
+    // rings (PF steps ahead) of census words and, with FUSE, of the other
+    // three directions' bytes; the loop body covers BLK steps (with FUSE one
+    // 4-step WTA block)
     constexpr int PF = FUSE ? FSGM_PF_F : FSGM_PF_V;
     constexpr int BLK = (PF > 4) ? PF : 4;
     static_assert(BLK % PF == 0, "the loop body must hold whole ring turns");
