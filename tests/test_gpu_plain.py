@@ -18,7 +18,8 @@ from fasterstereocuda_amd.stereo import CuSGMOption, StereoCuda
 pytestmark = pytest.mark.gpu
 
 # (W, H, D, min_d, cs_type, p1, p2): the 450x375 workload, ragged sizes,
-# every supported D, the 5x5 census, positive / negative min_d, P1 > 127.
+# every supported D, the 5x5 census, positive / negative min_d, P1 > 127,
+# and large penalties (path values past 2047 and past 8 bits of P2).
 CASES = [
     (450, 375, 64, 0, 1, 10, 150),
     (97, 61, 64, 0, 1, 10, 150),
@@ -30,6 +31,9 @@ CASES = [
     (90, 30, 64, 0, 1, 200, 300),
     (600, 21, 512, 0, 1, 10, 150),
     (77, 9, 128, 3, 0, 10, 150),
+    (140, 40, 64, 0, 1, 83, 1900),
+    (140, 40, 128, 0, 1, 84, 1900),
+    (120, 36, 64, 0, 1, 10, 5000),
 ]
 UNIQ, LR_THR = 0.95, 1.0
 
