@@ -550,8 +550,10 @@ int sgm_host_free(void* ptr) {
 // StereoCudaImpl::GetRoiFromDispMap (StereoCudaImpl.cpp:585-628): first and
 // last valid row by a raster scan, first and last valid column by a column
 // scan.
+// A map with no valid pixel leaves x and w as they were, as the reference's
+// scan does; an empty size leaves the whole ROI untouched.
 void sgm_roi_from_disparity(const float* disp, int32_t width, int32_t height, sgm_roi* roi) {
-    if (!disp || !roi) return;
+    if (!disp || !roi || width <= 0 || height <= 0) return;
     const int64_t size = (int64_t)width * height;
     int64_t n = 0;
     for (n = 0; n < size; ++n)

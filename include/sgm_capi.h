@@ -129,7 +129,8 @@ int sgm_host_alloc(void** ptr, size_t size);
 int sgm_host_free(void* ptr);
 
 /* Replaces StereoCuda::GetRoiFromDispMap (StereoCuda.h:84,
- * StereoCudaImpl.cpp:585-628): host-side scan. */
+ * StereoCudaImpl.cpp:585-628): host-side scan.  A NULL pointer or a size
+ * <= 0 leaves `roi` untouched; a map without valid pixels leaves x and w. */
 void sgm_roi_from_disparity(const float* disp, int32_t width, int32_t height, sgm_roi* roi);
 
 /* Where the aggregation takes its matching cost (no reference counterpart;

@@ -10,6 +10,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from fasterstereocuda_amd import _lib
@@ -152,6 +153,14 @@ def test_uninitialised_handle_and_constants():
     assert StereoCuda.get_invad_float() == -999999.0
     assert StereoCuda.get_invad_short() == -32768
     assert StereoCuda.get_level_range() == 32
+
+
+def test_roi_from_disparity_empty_size_is_a_no_op():
+    roi = _lib.SgmRoi(1, 2, 3, 4)
+    d = np.zeros(4, np.float32)
+    _lib.lib().sgm_roi_from_disparity(d.ctypes.data, 0, 4, ctypes.byref(roi))
+    _lib.lib().sgm_roi_from_disparity(d.ctypes.data, 4, -1, ctypes.byref(roi))
+    assert (roi.x, roi.y, roi.w, roi.h) == (1, 2, 3, 4)
 
 
 def test_roi_from_disparity_host_scan():
