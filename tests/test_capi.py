@@ -155,6 +155,16 @@ def test_uninitialised_handle_and_constants():
     assert StereoCuda.get_level_range() == 32
 
 
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """No CPU fallback: without the HIP library every entry point raises."""
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libfsgm.so"))
+    with pytest.raises(ImportError, match="no CPU fallback"):
+        _lib.lib()
+    with pytest.raises(ImportError):
+        StereoCuda().Init(64, 32, 0, 64, CuSGMOption.benchmark())
+
+
 def test_roi_from_disparity_empty_size_is_a_no_op():
     roi = _lib.SgmRoi(1, 2, 3, 4)
     d = np.zeros(4, np.float32)
