@@ -3,6 +3,7 @@ to end.  Integer stages (census, cost, per-path aggregation) must match bit
 for bit; the float disparity maps must match bit for bit as well (the HIP
 kernels reproduce the reference's float steps exactly), which is stricter
 than the 1e-5 px bound on subpixel output."""
+import ctypes
 import os
 import subprocess
 import tempfile
@@ -12,7 +13,7 @@ import pytest
 import torch
 
 import oracle
-from fasterstereocuda_amd import scenes, stages
+from fasterstereocuda_amd import _lib, scenes, stages
 from fasterstereocuda_amd.stereo import CuSGMOption, StereoCuda
 
 pytestmark = pytest.mark.gpu
@@ -167,6 +168,60 @@ def test_driving_full_size(cuda):
     assert st.match_device(Ld, Rd, od)
     torch.cuda.synchronize()
     _eq_float(od.cpu().numpy(), exp_l, "device entry point")
+
+
+def test_set_min_disparity_rebuilds_the_plan(cuda):
+    """SetMinDisparity after Init (StereoCuda.h:71-72): each later Match uses
+    the new minimum; positive, negative and back to 0 on one handle."""
+    W, H, D = 200, 60, 64
+    left, right = _pair(W, H, D, seed=11)
+    st = StereoCuda()
+    assert st.Init(W, H, 0, D, CuSGMOption.benchmark()), st.last_error
+    for md in (9, -6, 0):
+        print(f"[min disparity] {md}", flush=True)
+        st.SetMinDisparity(md)
+        exp_l, exp_r = oracle.match(left, right, md, D, uniqueness=UNIQ, lr_threshold=LR_THR)
+        dl = np.empty((H, W), np.float32)
+        dr = np.empty((H, W), np.float32)
+        assert st.Match(left, right, dl, dr), st.last_error
+        _eq_float(dl, exp_l, f"left map, min_d {md}")
+        _eq_float(dr, exp_r, f"right map, min_d {md}")
+    assert st.GetDispartyRange() == D
+
+
+def test_match_from_page_locked_buffers(cuda):
+    """The reference's calling pattern: images and maps in page-locked host
+    memory from MallocPageLockedPtr (StereoCuda.h:94-97, sgm_host_alloc)."""
+    W, H, D = 450, 375, 64
+    left, right = _pair(W, H, D, seed=3)
+    lib = _lib.lib()
+    ptrs = []
+
+    def pinned(dtype):
+        nbytes = H * W * np.dtype(dtype).itemsize
+        p = ctypes.c_void_p()
+        assert lib.sgm_host_alloc(ctypes.byref(p), nbytes) == 0, _lib.last_error()
+        ptrs.append(p)
+        raw = (ctypes.c_uint8 * nbytes).from_address(p.value)
+        return np.ctypeslib.as_array(raw).view(dtype).reshape(H, W)
+
+    try:
+        L, R = pinned(np.uint8), pinned(np.uint8)
+        dl, dr = pinned(np.float32), pinned(np.float32)
+        L[:] = left
+        R[:] = right
+        st = StereoCuda()
+        assert st.Init(W, H, 0, D, CuSGMOption.benchmark()), st.last_error
+        print("[match] page-locked 450x375", flush=True)
+        assert st.Match(L, R, dl, dr), st.last_error
+        got_l, got_r = dl.copy(), dr.copy()
+        st.Release()
+    finally:
+        for p in ptrs:
+            assert lib.sgm_host_free(p) == 0
+    exp_l, exp_r = oracle.match(left, right, 0, D, uniqueness=UNIQ, lr_threshold=LR_THR)
+    _eq_float(got_l, exp_l, "left map")
+    _eq_float(got_r, exp_r, "right map")
 
 
 def test_cpp_dropin_demo(cuda, tmp_path):
