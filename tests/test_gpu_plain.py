@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 
 # (W, H, D, min_d, cs_type, p1, p2): the 450x375 workload, ragged sizes,
 # every supported D, the 5x5 census, positive / negative min_d, P1 > 127,
-# and large penalties (path values past 2047 and past 8 bits of P2).
+# large penalties (path values past 2047 and past 8 bits of P2), and images
+# narrower than D and smaller than one scanline group.
 CASES = [
     (450, 375, 64, 0, 1, 10, 150),
     (97, 61, 64, 0, 1, 10, 150),
@@ -35,6 +36,8 @@ CASES = [
     (140, 40, 64, 0, 1, 83, 1900),
     (140, 40, 128, 0, 1, 84, 1900),
     (120, 36, 64, 0, 1, 10, 5000),
+    (12, 8, 64, 0, 1, 10, 150),
+    (20, 12, 256, 3, 0, 10, 150),
 ]
 UNIQ, LR_THR = 0.95, 1.0
 
