@@ -6,7 +6,7 @@ import pytest
 import torch
 
 import oracle
-from fasterstereocuda_amd import scenes
+from fasterstereocuda_amd import _lib, scenes
 from fasterstereocuda_amd.batch import SgmBatch
 from fasterstereocuda_amd.stereo import CuSGMOption, StereoCuda
 
@@ -50,6 +50,39 @@ def test_uneven_batch_matches_plain_and_oracle(cuda, lr):
             el, er = oracle.match(sc.left, sc.right, m, d, do_lr_check=lr)
             np.testing.assert_array_equal(_bits(got_l), _bits(el), err_msg=f"pair {i} left vs oracle")
             np.testing.assert_array_equal(_bits(got_r), _bits(er), err_msg=f"pair {i} right vs oracle")
+
+
+@pytest.mark.parametrize("mode", [2, 1], ids=["on-the-fly", "stored"])
+@pytest.mark.parametrize("specs", [
+    [(12, 8, 64, 0), (200, 40, 512, 0), (97, 33, 128, 2)],
+    [(12, 8, 64, 0), (20, 12, 256, 3), (150, 50, 64, -4)],
+], ids=["with-d512", "tiny"])
+def test_extreme_batches_match_oracle(cuda, specs, mode):
+    """Batches mixing a tiny image, D = 512 (no on-the-fly kernel: the batch
+    keeps the stored cost) and +-min_d, pair by pair against the oracle."""
+    lib = _lib.lib()
+    before = lib.sgm_get_cost_mode()
+    assert lib.sgm_set_cost_mode(mode) == 0
+    try:
+        opt = CuSGMOption.benchmark()
+        scs = [scenes.cone_like(20 + i, H=h, W=w, D=d) for i, (w, h, d, _) in enumerate(specs)]
+        pairs = [{"left": torch.from_numpy(sc.left).to(cuda), "right": torch.from_numpy(sc.right).to(cuda),
+                  "disp_left": torch.full((h, w), 7.0, device=cuda),
+                  "disp_right": torch.full((h, w), 7.0, device=cuda),
+                  "min_disparity": m, "disp_range": d}
+                 for sc, (w, h, d, m) in zip(scs, specs)]
+        print(f"[batch] extreme {specs} mode={mode}", flush=True)
+        b = SgmBatch(pairs, opt)
+        b.run()
+        torch.cuda.synchronize()
+    finally:
+        assert lib.sgm_set_cost_mode(before) == 0
+    for i, (sc, (w, h, d, m)) in enumerate(zip(scs, specs)):
+        el, er = oracle.match(sc.left, sc.right, m, d)
+        np.testing.assert_array_equal(_bits(pairs[i]["disp_left"].cpu().numpy()), _bits(el),
+                                      err_msg=f"pair {i} left vs oracle")
+        np.testing.assert_array_equal(_bits(pairs[i]["disp_right"].cpu().numpy()), _bits(er),
+                                      err_msg=f"pair {i} right vs oracle")
 
 
 def test_batch_stage_timing(cuda):
