@@ -96,6 +96,21 @@ def test_pyramid_matches_oracle(cuda, W, H, D, layers, lr, cs):
     np.testing.assert_array_equal(_bits(again), _bits(got))
 
 
+@pytest.mark.parametrize("W,H,dmin,dmax,layers,lr,cs", [(517, 389, 12, 160, 4, 16, 1),
+                                                        (450, 375, -16, 100, 3, 32, 0)])
+def test_pyramid_disparity_interval_matches_oracle(cuda, W, H, dmin, dmax, layers, lr, cs):
+    """A disparity interval that does not start at 0, divided down to the
+    coarsest of 3 or 4 levels (FasterStereoCuda.cpp:265-281)."""
+    print(f"[pyramid] {W}x{H} [{dmin}, {dmax}) layers{layers} lr{lr} cs{cs}", flush=True)
+    p = scenes.cone_like(9, H=H, W=W, D=128, workload=2)
+    opt = CuSGMOption.benchmark(cs_type=cs)
+    pyr = SgmPyramid(W, H, dmin, dmax, layers, lr, opt)
+    got = pyr.match(p.left, p.right)
+    exp = oracle.pyramid(p.left, p.right, dmin, dmax, layers, lr, cs_type=cs)
+    np.testing.assert_array_equal(_bits(got), _bits(exp))
+    assert (got != oracle.INVALID).mean() > 0.2
+
+
 def test_building_4800x3409_pyramid(cuda):
     """The large-image workload: 4800x3409, D = 256, 3 levels, window 16."""
     p = scenes.building(0)
