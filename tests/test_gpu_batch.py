@@ -85,6 +85,28 @@ def test_extreme_batches_match_oracle(cuda, specs, mode):
                                       err_msg=f"pair {i} right vs oracle")
 
 
+def test_driving_batch_full_size(cuda):
+    """The benchmark's own configuration: 32 driving pairs of 1242x375,
+    D = 64, LR check and median, batched with the default cost mode (the
+    cost computed on the fly at this size), every map against the oracle."""
+    pairs_np = scenes.driving_batch(32)
+    opt = CuSGMOption.benchmark()
+    pairs = [{"left": torch.from_numpy(p.left).to(cuda), "right": torch.from_numpy(p.right).to(cuda),
+              "disp_left": torch.full((375, 1242), 7.0, device=cuda),
+              "disp_right": torch.full((375, 1242), 7.0, device=cuda),
+              "min_disparity": 0, "disp_range": 64} for p in pairs_np]
+    print("[batch] 32 driving pairs 1242x375", flush=True)
+    b = SgmBatch(pairs, opt)
+    b.run()
+    torch.cuda.synchronize()
+    for i, p in enumerate(pairs_np):
+        el, er = oracle.match(p.left, p.right, 0, 64)
+        np.testing.assert_array_equal(_bits(pairs[i]["disp_left"].cpu().numpy()), _bits(el),
+                                      err_msg=f"pair {i} left vs oracle")
+        np.testing.assert_array_equal(_bits(pairs[i]["disp_right"].cpu().numpy()), _bits(er),
+                                      err_msg=f"pair {i} right vs oracle")
+
+
 def test_batch_stage_timing(cuda):
     sc = scenes.driving(0)
     h, w = sc.left.shape
