@@ -550,8 +550,8 @@ __device__ __forceinline__ float raw_at(const float* m, const SgmPost& P, int x,
     return (x < P.roi_x) ? kInvalidDisp : gld(m + (int64_t)y * P.W + x);
 }
 
-__device__ __forceinline__ float lr_value(const SgmPost& P, int x, int y) {
-    const float dl = raw_at(P.raw_l, P, x, y);
+// the LR rule for the left value dl of (x, y)
+__device__ __forceinline__ float lr_finish(const SgmPost& P, int x, int y, float dl) {
     if (!P.do_lr || dl == kInvalidDisp) return dl;
     const float xrf = (float)x - dl;
     const int xr = (xrf > 0) ? (int)((double)xrf + 0.5) : (int)((double)xrf - 0.5);
@@ -587,10 +587,23 @@ __global__ __launch_bounds__(256) void post_kernel(const SgmPost* __restrict__ p
     const int tiles_x = (P.W + 15) / 16;
     const int x0 = (local % tiles_x) * 16, y0 = (local / tiles_x) * 16;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < 18 * 18; idx += 256) {
-        const int r = idx / 18, c = idx % 18;
-        const int yy = y0 - 1 + r, xx = x0 - 1 + c;
-        t[r][c] = (xx >= 0 && yy >= 0 && xx < P.W && yy < P.H) ? lr_value(P, xx, yy) : kInvalidDisp;
+    // two tile entries per thread (18 x 18 = 256 + 68): both left-map loads
+    // are issued before either right-map load that depends on them
+    float dl[2];
+    int ex[2], ey[2];
+    bool in[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int idx = tid + 256 * u;
+        ex[u] = x0 - 1 + idx % 18;
+        ey[u] = y0 - 1 + idx / 18;
+        in[u] = idx < 18 * 18 && ex[u] >= 0 && ey[u] >= 0 && ex[u] < P.W && ey[u] < P.H;
+        dl[u] = in[u] ? raw_at(P.raw_l, P, ex[u], ey[u]) : kInvalidDisp;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int idx = tid + 256 * u;
+        if (idx < 18 * 18) t[idx / 18][idx % 18] = in[u] ? lr_finish(P, ex[u], ey[u], dl[u]) : kInvalidDisp;
     }
     __syncthreads();
     const int tx = tid & 15, ty = tid >> 4;
