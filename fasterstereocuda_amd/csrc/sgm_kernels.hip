@@ -75,11 +75,22 @@ __global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ 
     const int tiles_x = (J.W + 63) / 64;
     const int x0 = (local % tiles_x) * 64, y0 = (local / tiles_x) * 16;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < 22 * 72; idx += 256) {
+    // 22 x 72 bytes over 256 threads: every load is issued before the first
+    // LDS store, so the 7 loads of a thread overlap
+    constexpr int kTileN = 22 * 72, kPer = (kTileN + 255) / 256;
+    uint8_t px[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int idx = tid + 256 * u;
         const int r = idx / 72, c = idx % 72;
         const int yy = min(max(y0 - 3 + r, 0), J.H - 1);
         const int xx = min(max(x0 - 4 + c, 0), J.W - 1);
-        tile[r][c] = gld(J.img + (int64_t)yy * J.W + xx);
+        px[u] = (idx < kTileN) ? gld(J.img + (int64_t)yy * J.W + xx) : (uint8_t)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int idx = tid + 256 * u;
+        if (idx < kTileN) tile[idx / 72][idx % 72] = px[u];
     }
     __syncthreads();
     const int lx = tid & 63, ly = tid >> 6;
