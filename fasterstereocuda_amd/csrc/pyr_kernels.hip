@@ -110,8 +110,13 @@ __global__ __launch_bounds__(256) void cost_init_kernel(PyrLevel a) {
 }
 
 // Aggregation over the per-pixel window (cusgm_ca.cu:1181-1339), one
-// disparity per lane, SPW = 64 / LR scanlines per wave; see the oracle for the
-// step rules.  A scanline is walked in blocks of K = LR steps.  For the loads
+// disparity per lane, SPW = 64 / LR scanlines per wave.  Step rules, with b
+// and b' the window bases of this and the previous pixel: the neighbour
+// terms read the previous L at window index t + b - 1 - b' and t + b + 1 - b'
+// plus P1 (127 outside [0, LR)), the same-d term is L(t) unshifted, P2 is
+// max(P1, P2 / max(1, |b' - b + 1|)), and a pixel with an invalid base keeps
+// the previous L and b' and sets m' = 127.  The first pixel of a vertical
+// scanline takes b' from row 0.  A scanline is walked in blocks of K = LR steps.  For the loads
 // and stores lane j stands for pixel (line j / K, step j % K) of a block: it
 // fetches that pixel's whole window (LR cost bytes) and base disparity one
 // block ahead, and the block passes through a per-wave LDS area, from which
