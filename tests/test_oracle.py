@@ -12,6 +12,10 @@ arithmetic, against closed forms, or against a library routine:
   lrcheck    numpy restatement + hand cases
   median     scipy.ndimage.median_filter
   pipeline   synthetic scenes with known disparity (error rates)
+  pyramid    init propagation, window WTA and window aggregation (P2 divisor,
+             shifted neighbours, vertical seed row) by hand-worked cases;
+             dilation by scipy.ndimage.maximum_filter; the level schedule by
+             composing the pinned stage functions
 """
 import numpy as np
 import pytest
@@ -441,6 +445,203 @@ def test_wta_init_numpy():
             den = max(np.float32(1), c1 + c3 - np.float32(2) * c2)
             exp[y, x] = np.float32(ds + int(init[y, x])) + (c1 - c3 + den) / (np.float32(2) * den)
     np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+# Hand cases: every expected value below was worked out on paper from the
+# reference kernels' rules, not by running a formula, so a dropped term, a
+# flipped sign or a wrong index in the oracle fails them.
+
+def test_init_next_hand_cases():
+    """FasterStereoCuda.cpp:138-166 on a 5x5 level from a 2x2 coarse map
+    (window 32, half 16): even rows 2d - 16 from the coarse pixel (i/2, j/2),
+    odd rows and the last row copy the row above, INVALID -> -32768, and the
+    odd width's last column reads the coarse column 1 (A18)."""
+    coarse = np.array([[1.0, 2.0], [3.0, INV]], np.float32)
+    r0 = [-14, -14, -12, -12, -12]
+    r2 = [-10, -10, -32768, -32768, -32768]
+    exp = np.array([r0, r0, r2, r2, r2], np.int16)
+    np.testing.assert_array_equal(oracle.init_next(coarse, 5, 5, 32), exp)
+    # the short cast truncates toward zero: 2*3.7 - 8 = -0.6 -> 0 (floor: -1),
+    # 2*10.3 - 8 = 12.6 -> 12, 2*8.25 - 8 = 8.5 -> 8, 2*(-0.5) - 8 = -9
+    coarse = np.array([[3.7, 10.3], [8.25, -0.5]], np.float32)
+    exp = np.array([[0, 0, 12, 12], [0, 0, 12, 12], [8, 8, -9, -9], [8, 8, -9, -9]], np.int16)
+    np.testing.assert_array_equal(oracle.init_next(coarse, 4, 4, 16), exp)
+
+
+def test_dilation_matches_scipy_maximum_filter():
+    """cusgm_df.cu:198-222: an INVALID pixel strictly inside the border takes
+    the maximum of its window (scipy.ndimage.maximum_filter); every other
+    pixel, INVALID border pixels included, is copied."""
+    rng = np.random.default_rng(13)
+    H, W = 23, 31
+    d = rng.uniform(-20, 90, size=(H, W)).astype(np.float32)
+    d[rng.random(d.shape) < 0.45] = INV
+    d[4:15, 4:15] = INV                     # a hole wider than the 7x7 window
+    for wnd in (3, 5, 7):
+        r = (wnd - 1) // 2
+        mx = ndimage.maximum_filter(d, size=wnd, mode="nearest")
+        got = oracle.dilation(d, wnd)
+        inner = np.zeros_like(d, bool)
+        inner[r + 1:H - r, r + 1:W - r] = True
+        fill = inner & (d == INV)
+        np.testing.assert_array_equal(got[fill], mx[fill])
+        np.testing.assert_array_equal(got[~fill], d[~fill])
+        assert (got[fill] == INV).any() and (got[fill] != INV).any()
+
+
+def _wta_init_case(S, b, uniqueness=1.0):
+    """S: (H, W, LR) sums laid out over four paths (1, 2, 3 and the rest)."""
+    S = np.asarray(S, np.int64)
+    paths = [(S - 6).astype(np.uint8), np.full(S.shape, 1, np.uint8),
+             np.full(S.shape, 2, np.uint8), np.full(S.shape, 3, np.uint8)]
+    return oracle.wta_init(paths, b, uniqueness)
+
+
+def test_wta_init_hand_cases():
+    """cusgm_ca.cu:3425-3512 with window 8 at row y = 5 of a 20x12 level:
+    border x < 5 or x > W - 5 -> INVALID; INVALID base -> INVALID; d* the
+    smallest argmin (A1); d* <= 1 or d* >= LR - 1 -> INVALID; disp = d* + b +
+    (c1 - c3 + den) / (2 den), den = max(1, c1 + c3 - 2 c2)."""
+    H, W, LR = 12, 20, 8
+    S = np.full((H, W, LR), 99, np.int64)
+    b = np.zeros((H, W), np.int16)
+    def put(x, vals, base, y=5):
+        S[y, x] = vals
+        b[y, x] = base
+    put(5, [19, 19, 13, 15, 19, 19, 19, 19], 10)     # d* 2, c1 19 c3 15 c2 13: den 8, bias 12/16
+    put(6, [19, 13, 19, 19, 19, 19, 19, 19], 10)     # d* 1 -> INVALID
+    put(7, [19, 19, 19, 19, 19, 19, 19, 13], 10)     # d* 7 = LR - 1 -> INVALID
+    put(8, [19, 19, 19, 19, 19, 19, 13, 15], -5)     # d* 6 = LR - 2: 6 - 5 + 0.75
+    put(9, [13, 19, 19, 19, 19, 19, 19, 19], 10)     # d* 0 -> INVALID
+    put(10, [19, 19, 19, 13, 19, 19, 19, 19], -32768)  # INVALID base
+    put(11, [19, 19, 13, 15, 13, 19, 19, 19], 0)     # tie at 2 and 4: d* 2, bias 0.75
+    put(12, [19, 19, 19, 16, 12, 16, 19, 19], 1)     # c1 = c3: den 8, bias 0.5 -> 5.5
+    put(4, [19, 19, 13, 15, 19, 19, 19, 19], 10)     # border column
+    put(16, [19, 19, 13, 15, 19, 19, 19, 19], 10)    # x > W - 5
+    put(15, [19, 19, 13, 15, 19, 19, 19, 19], 10)    # x = W - 5: inside
+    put(7, [19, 19, 13, 15, 19, 19, 19, 19], 3, y=7)   # y = H - 5: inside
+    put(7, [19, 19, 13, 15, 19, 19, 19, 19], 3, y=8)   # y > H - 5
+    put(7, [19, 19, 13, 15, 19, 19, 19, 19], 3, y=4)   # y < 5
+    got = _wta_init_case(S, b)
+    exp = np.full((H, W), INV, np.float32)
+    exp[5, 5] = 12.75
+    exp[5, 8] = 1.75
+    exp[5, 11] = 2.75
+    exp[5, 12] = 5.5
+    exp[5, 15] = 12.75
+    exp[7, 7] = 5.75
+    # every other pixel has S = 99 everywhere: d* = 0 -> INVALID
+    np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+def test_wta_init_uniqueness_hand_case():
+    """uniq = min(c1 - c2, c3 - c2) <= c2 (1 - u) -> INVALID.  c2 = 100,
+    c1 = c3 = 104: 4 <= 100 * 0.05 fails at u = 0.95, passes at u = 1.0;
+    c1 = c3 = 106 (6 > 5) passes both.  bias 0.5 from c1 = c3."""
+    H, W, LR = 12, 20, 8
+    S = np.full((H, W, LR), 255, np.int64)
+    S[:, :, 0] = 0                       # filler pixels: d* = 0 -> INVALID
+    b = np.zeros((H, W), np.int16)
+    S[6, 6] = [200, 200, 104, 100, 104, 200, 200, 200]
+    S[6, 7] = [200, 200, 106, 100, 106, 200, 200, 200]
+    b[6, 6] = b[6, 7] = 20
+    strict = _wta_init_case(S, b, 0.95)
+    loose = _wta_init_case(S, b, 1.0)
+    assert strict[6, 6] == INV and loose[6, 6] == np.float32(23.5)
+    assert strict[6, 7] == np.float32(23.5) and loose[6, 7] == np.float32(23.5)
+
+
+def _agg_row(c0, c1, b0, b1, p1=2, p2=60):
+    """left->right over a one-row, two-pixel level with window 4."""
+    cost = np.array([[c0, c1]], np.uint8)
+    init = np.array([[b0, b1]], np.int16)
+    return oracle.aggregate_init(cost, init, 1, p1, p2)[0]
+
+
+def test_aggregate_init_p2_divisor_hand_cases():
+    """cusgm_ca.cu:1181-1262, window 4, P1 = 2, P2 = 60.  Pixel 0: b' = 10,
+    L' = C = [30, 90, 100, 110], m' = 30.  Pixel 1: C = [5, 6, 7, 8] and
+    L(t) = C(t) + min(L'(t), B-, B+, m' + max(P1, P2 / max(1, |b' - b + 1|))) - m'
+    with B- = L'(t + b - 1 - b') + P1, B+ = L'(t + b + 1 - b') + P1 (127
+    outside the window).  Worked cases (divisor |b' - b + 1| in brackets):
+      b = 11 [0 -> 1]: P2' 60, B- = L'(t) + 2, B+ = L'(t + 2) + 2 -> [5, 66, 67, 68]
+      b = 13 [2]:      P2' 30, B- = L'(t + 2) + 2, B+ = 127      -> [5, 36, 37, 38]
+      b = 16 [5]:      P2' 12, both 127                           -> [5, 18, 19, 20]
+      b = 9  [2]:      P2' 30, B- = L'(t - 2) + 2, B+ = L'(t) + 2 -> [5, 36, 9, 38]
+    (|b' - b| + 1 would give P2' 30 / 15 / 8 at b = 11 / 13 / 16.)"""
+    c0, c1 = [30, 90, 100, 110], [5, 6, 7, 8]
+    for b1, exp in ((11, [5, 66, 67, 68]), (13, [5, 36, 37, 38]), (16, [5, 18, 19, 20]),
+                    (9, [5, 36, 9, 38])):
+        got = _agg_row(c0, c1, 10, b1)
+        np.testing.assert_array_equal(got[0], c0)           # first pixel stored as C (A8)
+        np.testing.assert_array_equal(got[1], exp, err_msg=f"b = {b1}")
+
+
+def test_aggregate_init_vertical_seed_hand_case():
+    """cusgm_ca.cu:1291 (A11): both vertical directions take b' of their
+    first step from row 0.  One column of 3 rows, window 4, P1 = 2, P2 = 60,
+    b = [20, 10, 10], C rows [50, 40, 45, 60], [5, 6, 7, 8], [30, 90, 100, 110].
+    bottom->top: row 2 stored as C, m' = 30, b' = 20 (row 0, not row 2's 10):
+      row 1: |20 - 10 + 1| = 11 -> P2' max(2, 5) = 5, B-/B+ outside -> [5, 11, 12, 13]
+             (with b' = 10 it would be [5, 8, 67, 68])
+      row 0: b' = 10, |10 - 20 + 1| = 9 -> P2' 6, m' = 5 -> [50, 46, 51, 66]
+    top->bottom: row 0 = C, m' = 40, b' = 20:
+      row 1: P2' 5 -> [10, 6, 12, 13];  row 2: b' = 10 = b, P2' 60, m' = 6,
+      B- = L'(t - 1) + 2, B+ = L'(t + 1) + 2 -> [32, 90, 102, 117]"""
+    cost = np.array([[[50, 40, 45, 60]], [[5, 6, 7, 8]], [[30, 90, 100, 110]]], np.uint8)
+    init = np.array([[20], [10], [10]], np.int16)
+    du = oracle.aggregate_init(cost, init, 2, 2, 60)[:, 0]
+    np.testing.assert_array_equal(du, [[50, 46, 51, 66], [5, 11, 12, 13], [30, 90, 100, 110]])
+    ud = oracle.aggregate_init(cost, init, 0, 2, 60)[:, 0]
+    np.testing.assert_array_equal(ud, [[50, 40, 45, 60], [10, 6, 12, 13], [32, 90, 102, 117]])
+
+
+def _schedule(layer, num_layers, do_lr):
+    """SetSgmParameters (FasterStereoCuda.cpp:169-199): uniqueness 0.95 at
+    layer 0 and 1.0 above; with more than one layer the LR check, the peak
+    removal and the dilation happen at the coarsest layer only."""
+    top = layer == num_layers - 1
+    return dict(level0=layer == 0, lr=do_lr and top, dilate=top and num_layers > 1)
+
+
+def _compose_pyramid(left, right, dmin, drange, num_layers, u, u_above, ratio):
+    """HierSgmCudaImpl::Match (FasterStereoCuda.cpp:408-482) built from the
+    pinned stage functions: decimate, match the coarsest level (LR check,
+    median, dilation, peaks), then per finer level propagate, window-match
+    and median.  u_above is the uniqueness of the levels above 0."""
+    L, R = [left], [right]
+    for _ in range(1, num_layers):
+        L.append(oracle.zoom_half(L[-1]))
+        R.append(oracle.zoom_half(R[-1]))
+    top = num_layers - 1
+    s = _schedule(top, num_layers, True)
+    cur, _ = oracle.match(L[top], R[top], dmin, drange, uniqueness=u if s["level0"] else u_above,
+                          do_lr_check=s["lr"])
+    h, w = cur.shape
+    if s["dilate"]:
+        cur = oracle.dilation(cur, oracle.dilation_window(w, h))
+    cur, removed = oracle.remove_peaks(cur, 1, ratio)
+    for n in range(top - 1, -1, -1):
+        s = _schedule(n, num_layers, True)
+        assert not s["lr"] and not s["dilate"]
+        h, w = L[n].shape
+        init = oracle.init_next(cur, w, h, 16)
+        cur = oracle.match_init(L[n], R[n], init, 16, uniqueness=u if s["level0"] else u_above)
+    return cur, removed
+
+
+@pytest.mark.parametrize("num_layers", [2, 3])
+def test_pyramid_composes_the_level_schedule(num_layers):
+    p = scenes.cone_like(3, H=96, W=160, D=64)
+    u, ratio = 0.95, 0.02
+    got = oracle.pyramid_at(p.left, p.right, 0, 64, num_layers, 16, uniqueness=u,
+                            peaks_ratio=ratio)
+    exp, removed = _compose_pyramid(p.left, p.right, 0, 64, num_layers, u, 1.0, ratio)
+    assert removed > 0
+    np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+    # the schedule is not vacuous: uniqueness 0.95 above level 0 changes the result
+    alt, _ = _compose_pyramid(p.left, p.right, 0, 64, num_layers, u, u, ratio)
+    assert not np.array_equal(alt.view(np.uint32), got.view(np.uint32))
 
 
 def test_pyramid_recovers_ground_truth():
