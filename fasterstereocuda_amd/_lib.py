@@ -137,6 +137,8 @@ SIGNATURES = {
     "sgm_stage_census": (ctypes.c_int, [P, I32, I32, I32, I32, P, P]),
     "sgm_stage_cost": (ctypes.c_int, [P, P, I32, I32, I32, I32, I32, I32, P, P]),
     "sgm_stage_aggregate": (ctypes.c_int, [P, I32, I32, I32, I32, I32, I32, PP4, P]),
+    "sgm_stage_aggregate_wta_otf": (ctypes.c_int, [P, P, I32, I32, I32, I32, I32, I32, I32, I32,
+                                                   F32, PP4, P, P]),
     "sgm_stage_wta": (ctypes.c_int, [PP4, I32, I32, I32, I32, F32, P, P]),
     "sgm_stage_post": (ctypes.c_int, [P, P, I32, I32, I32, F32, I32, P, P, P]),
 }

@@ -34,6 +34,10 @@ class SgmBatch:
                 if not t.is_cuda or t.dtype != dt or tuple(t.shape) != (H, W) or not t.is_contiguous():
                     raise ValueError(f"pair {i}: {k} must be a contiguous {dt} CUDA tensor [{H}, {W}]")
             dr = p.get("disp_right")
+            if dr is not None and (not dr.is_cuda or dr.dtype != torch.float32
+                                   or tuple(dr.shape) != (H, W) or not dr.is_contiguous()):
+                raise ValueError(f"pair {i}: disp_right must be a contiguous float32 CUDA tensor "
+                                 f"[{H}, {W}]")
             arr[i] = SgmPairDesc(W, H, int(p.get("min_disparity", 0)), int(p["disp_range"]),
                                  p["left"].data_ptr(), p["right"].data_ptr(),
                                  p["disp_left"].data_ptr(), None if dr is None else dr.data_ptr())

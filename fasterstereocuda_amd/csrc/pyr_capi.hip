@@ -107,9 +107,17 @@ bool pyramid_args_ok(int32_t width, int32_t height, int32_t num_layers, int32_t 
         set_global_error(SGM_E_INVALID_ARG, "pyramid: bad options");
         return false;
     }
-    if (opt->post_filter_type != 0 || opt->morphology_type != 0 || !opt->using_constant_p2) {
+    if (opt->post_filter_type != 0 || opt->morphology_type != 0) {
+        set_global_error(SGM_E_UNSUPPORTED, "pyramid: post filters and morphology are not built");
+        return false;
+    }
+    // the finer levels aggregate around initial disparities, which the
+    // reference does only with constant P2 (cusgm_ca.cu:4052-4077); a single
+    // level is the plain path, which ignores the flag
+    if (layers > 1 && !opt->using_constant_p2) {
         set_global_error(SGM_E_UNSUPPORTED,
-                         "pyramid: post filters, morphology and adaptive P2 are not built");
+                         "pyramid: levels above one need using_constant_p2 (the reference's "
+                         "window aggregation has no adaptive-P2 form)");
         return false;
     }
     if (opt->do_remove_peaks && (int64_t)width * height > INT32_MAX) {

@@ -100,8 +100,9 @@ int check_option(const sgm_option* o, std::string* err, bool allow_peaks) {
         return fail(err, SGM_E_UNSUPPORTED, "post filters (Gauss/bilateral) are not built");
     if (o->morphology_type != 0)
         return fail(err, SGM_E_UNSUPPORTED, "morphology filters are not built");
-    if (!o->using_constant_p2)
-        return fail(err, SGM_E_UNSUPPORTED, "intensity-adaptive P2 is not built");
+    // using_constant_p2 is not checked: the reference reads it only when it
+    // aggregates with initial disparities (cusgm_ca.cu:4005-4008); the plain
+    // aggregation ignores it (cusgm_ca.cu:4080-4135), and so does this one
     return SGM_OK;
 }
 
@@ -259,6 +260,11 @@ struct sgm_handle {
     bool inited = false;
     int device = -1;
     int32_t W = 0, H = 0, min_d = 0, D = 0;
+    // D = LEVEL_RANGE (32): the handle matches around an initial disparity
+    // map only (window cost, window aggregation, window WTA), as the
+    // reference's finer pyramid levels do (FasterStereoCuda.cpp:282-289, 455-469)
+    bool window = false;
+    int16_t* init = nullptr;      // W x H initial disparities (window handles)
     sgm_option opt{};
     bool print_log = false;
     hipStream_t stream = nullptr;
@@ -276,7 +282,8 @@ namespace {
 
 size_t arena_bytes(int W, int H, int D) {
     const size_t npix = (size_t)W * H, ncell = npix * D;
-    return 2 * (align256(npix) + align256(npix * 8) + 5 * align256(ncell) + 2 * align256(npix * 4));
+    return 2 * (align256(npix) + align256(npix * 8) + 5 * align256(ncell) + 2 * align256(npix * 4)) +
+           align256(npix * 2);
 }
 
 void carve(sgm_handle* h) {
@@ -292,6 +299,7 @@ void carve(sgm_handle* h) {
         b.raw = reinterpret_cast<float*>(take(npix * 4));
         b.out = reinterpret_cast<float*>(take(npix * 4));
     }
+    h->init = reinterpret_cast<int16_t*>(take(npix * 2));
 }
 
 SgmJob make_job(const sgm_handle* h, int v) {
@@ -326,40 +334,81 @@ int build_handle_plan(sgm_handle* h) {
     p.do_lr = h->opt.do_lr_check ? 1 : 0;
     p.do_median = h->opt.do_median_filter ? 1 : 0;
     p.lr_thr = h->opt.lr_threshold;
+    if (h->window) {
+        // census of both views, then the median of the window WTA's map; the
+        // window pass covers every column and has no LR check
+        for (SgmJob& j : jobs) { j.D = 64; j.min_d = 0; j.roi_x = 0; }
+        p.roi_x = 0;
+        p.do_lr = 0;
+    }
     std::vector<SgmPost> posts{p};
     if (int rc = upload_plan(jobs, posts, &h->plan, &h->plan_block, &h->err)) return rc;
-    if (!h->opt.do_lr_check) restrict_matching(&h->plan, jobs, 1);
+    if (!h->opt.do_lr_check && !h->window) restrict_matching(&h->plan, jobs, 1);
     return SGM_OK;
 }
 
-// Runs the pipeline on the handle's own buffers.  With the LR check off the
-// right view is still censused (the left cost needs it) but not aggregated.
-int run_pipeline(sgm_handle* h, bool want_right, hipStream_t s) {
+// The window pass of a D = LEVEL_RANGE handle (StereoCudaImpl::Match with
+// init_disp_left, StereoCudaImpl.cpp:363-446; cusgm_cc.cu:173-237,
+// cusgm_ca.cu:1181-1339, 3425-3512): census of both views, window cost,
+// window aggregation, window WTA, median; marks as in launch_match.
+hipError_t launch_window(const sgm_handle* h, const StagePlan& plan, hipStream_t s,
+                         hipEvent_t* marks) {
+    auto mark = [&](int k) { return marks ? hipEventRecord(marks[k], s) : hipSuccess; };
+    PyrLevel a{};
+    a.census_l = h->view[0].census; a.census_r = h->view[1].census; a.init = h->init;
+    a.cost = h->view[0].cost;
+    for (int r = 0; r < 4; ++r) a.path[r] = h->view[0].path[r];
+    a.disp = h->view[0].raw;
+    a.w = h->W; a.h = h->H; a.level_range = h->D;
+    a.cs_type = h->opt.cs_type; a.p1 = h->opt.p1; a.p2 = h->opt.p2_init;
+    a.uniqueness = h->opt.unique_threshold;
+    hipError_t e;
+    if ((e = mark(0)) != hipSuccess || (e = launch_census(plan, s)) != hipSuccess) return e;
+    if ((e = mark(1)) != hipSuccess || (e = pyr_launch_cost_init(a, s)) != hipSuccess) return e;
+    if ((e = mark(2)) != hipSuccess || (e = pyr_launch_aggregate_init(a, s)) != hipSuccess) return e;
+    if ((e = mark(3)) != hipSuccess || (e = pyr_launch_wta_init(a, s)) != hipSuccess) return e;
+    if ((e = mark(4)) != hipSuccess || (e = launch_post(plan, s)) != hipSuccess) return e;
+    return mark(5);
+}
+
+// Runs the pipeline; `io` (may be null) binds caller buffers in place of the
+// handle's images and outputs.  With the LR check off the right view is still
+// censused (the left cost needs it) but not aggregated.
+int run_pipeline(sgm_handle* h, bool want_right, hipStream_t s, const IoBind* io = nullptr) {
     const bool lr = h->opt.do_lr_check != 0;
+    StagePlan plan = h->plan;
+    if (io) plan.io = *io;
+    float* const final_left = io ? io->out_l : h->view[0].out;
     // peak removal on the final left map, as StereoCudaImpl::Match does after
     // the filters (StereoCudaImpl.cpp:541-542)
     auto peaks = [&]() {
         return h->opt.do_remove_peaks
-                   ? launch_remove_peaks(h->view[0].out, h->peaks, h->W, h->H, peak_strips(),
+                   ? launch_remove_peaks(final_left, h->peaks, h->W, h->H, peak_strips(),
                                          h->opt.peaks_ratio_threshold, s)
                    : hipSuccess;
     };
+    auto run = [&](hipEvent_t* marks) {
+        return h->window ? launch_window(h, plan, s, marks)
+                         : launch_match(plan, want_right && lr, s, marks);
+    };
     if (!h->print_log) {
-        HIP_TRY(&h->err, launch_match(h->plan, want_right && lr, s, nullptr));
+        HIP_TRY(&h->err, run(nullptr));
         HIP_TRY(&h->err, peaks());
         return SGM_OK;
     }
     hipEvent_t ev[6];
     for (auto& e : ev) HIP_TRY(&h->err, hipEventCreate(&e));
-    HIP_TRY(&h->err, launch_match(h->plan, want_right && lr, s, ev));
+    HIP_TRY(&h->err, run(ev));
     HIP_TRY(&h->err, peaks());
     HIP_TRY(&h->err, hipEventSynchronize(ev[5]));
     static const char* names[5] = {"Census:", "Cost:", "Aggregate (3 paths):",
                                    "Aggregate + WTA:", "LRCheck/Filter:"};
+    static const char* wnames[5] = {"Census:", "Cost (window):", "Aggregate (window):",
+                                    "WTA (window):", "Filter:"};
     for (int k = 0; k < 5; ++k) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
-        std::printf("** %-24s %.3f ms\n", names[k], ms);
+        std::printf("** %-24s %.3f ms\n", h->window ? wnames[k] : names[k], ms);
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return SGM_OK;
@@ -369,6 +418,32 @@ int check_ready(sgm_handle* h) {
     if (!h) return fail(nullptr, SGM_E_INVALID_ARG, "handle is NULL");
     if (!h->inited) return fail(&h->err, SGM_E_NOT_INIT, "handle is not initialised");
     HIP_TRY(&h->err, hipSetDevice(h->device));
+    return SGM_OK;
+}
+
+// Which pass a match with (init != NULL) or without an initial disparity map
+// runs on this handle, or the error that refuses it.
+int check_match_kind(sgm_handle* h, const void* init, const sgm_roi* rl, const sgm_roi* rr) {
+    if (rl || rr) return fail(&h->err, SGM_E_UNSUPPORTED, "ROIs are not built");
+    if (!init) {
+        if (h->window)
+            return fail(&h->err, SGM_E_UNSUPPORTED,
+                        "a disp_range 32 handle matches with an initial disparity map only");
+        return SGM_OK;
+    }
+    if (!h->window)
+        return fail(&h->err, SGM_E_UNSUPPORTED,
+                    "initial disparities need disp_range 32 (LEVEL_RANGE); the reference's "
+                    "D = 64 kernels with initial disparities are not built");
+    if (h->opt.do_lr_check)
+        return fail(&h->err, SGM_E_UNSUPPORTED,
+                    "the LR check with initial disparities is not built");
+    if (!h->opt.using_constant_p2)
+        return fail(&h->err, SGM_E_UNSUPPORTED,
+                    "with initial disparities the reference aggregates only with constant P2");
+    if (h->min_d > 0)
+        return fail(&h->err, SGM_E_UNSUPPORTED,
+                    "initial disparities with min_disparity > 0 (a column offset) are not built");
     return SGM_OK;
 }
 
@@ -390,8 +465,10 @@ int sgm_init(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity
     // validate everything before touching the device
     if (width <= 0 || height <= 0)
         return fail(&h->err, SGM_E_INVALID_ARG, "image size must be positive");
-    if (!valid_range(disp_range))
-        return fail(&h->err, SGM_E_INVALID_ARG, "disp_range must be 64, 128, 256 or 512");
+    if (!valid_range(disp_range) && disp_range != SGM_LEVEL_RANGE)
+        return fail(&h->err, SGM_E_INVALID_ARG,
+                    "disp_range must be 64, 128, 256 or 512 (or 32 = LEVEL_RANGE for matching "
+                    "around initial disparities)");
     if (std::max(0, min_disparity) >= width)
         return fail(&h->err, SGM_E_INVALID_ARG, "min_disparity leaves no column to match");
     if (int rc = check_option(opt, &h->err, true)) return rc;
@@ -403,6 +480,7 @@ int sgm_init(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity
         return fail(&h->err, SGM_E_NO_DEVICE, "no HIP device");
     HIP_TRY(&h->err, hipGetDevice(&h->device));
     h->W = width; h->H = height; h->min_d = min_disparity; h->D = disp_range;
+    h->window = disp_range == SGM_LEVEL_RANGE;
     h->opt = *opt;
     h->print_log = print_log != 0;
     HIP_TRY(&h->err, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -422,17 +500,15 @@ int sgm_match_device(sgm_handle* h, const uint8_t* d_left, const uint8_t* d_righ
     if (int rc = check_ready(h)) return rc;
     if (!d_left || !d_right || !d_disp_left)
         return fail(&h->err, SGM_E_INVALID_ARG, "image or output pointer is NULL");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const size_t npix = (size_t)h->W * h->H;
-    HIP_TRY(&h->err, hipMemcpyAsync(h->view[0].img, d_left, npix, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(&h->err, hipMemcpyAsync(h->view[1].img, d_right, npix, hipMemcpyDeviceToDevice, s));
-    if (int rc = run_pipeline(h, d_disp_right != nullptr, s)) return rc;
-    HIP_TRY(&h->err, hipMemcpyAsync(d_disp_left, h->view[0].out, npix * 4,
-                                    hipMemcpyDeviceToDevice, s));
-    if (d_disp_right)
-        HIP_TRY(&h->err, hipMemcpyAsync(d_disp_right, h->view[1].out, npix * 4,
-                                        hipMemcpyDeviceToDevice, s));
-    return SGM_OK;
+    if (int rc = check_match_kind(h, nullptr, nullptr, nullptr)) return rc;
+    // the caller's buffers are read and written in place: no staging copies
+    IoBind io{};
+    io.img[0] = d_left;
+    io.img[1] = d_right;
+    io.out_l = d_disp_left;
+    io.out_r = d_disp_right;
+    io.on = 1;
+    return run_pipeline(h, d_disp_right != nullptr, static_cast<hipStream_t>(stream), &io);
 }
 
 int sgm_match(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
@@ -441,14 +517,13 @@ int sgm_match(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
     if (int rc = check_ready(h)) return rc;
     if (!img_left || !img_right || !disp_left)
         return fail(&h->err, SGM_E_INVALID_ARG, "image or output pointer is NULL");
-    if (init_disp_left)
-        return fail(&h->err, SGM_E_UNSUPPORTED,
-                    "initial disparities are used through the pyramid entry points");
-    if (roi_left || roi_right) return fail(&h->err, SGM_E_UNSUPPORTED, "ROIs are not built");
+    if (int rc = check_match_kind(h, init_disp_left, roi_left, roi_right)) return rc;
     const size_t npix = (size_t)h->W * h->H;
     hipStream_t s = h->stream;
     HIP_TRY(&h->err, hipMemcpyAsync(h->view[0].img, img_left, npix, hipMemcpyHostToDevice, s));
     HIP_TRY(&h->err, hipMemcpyAsync(h->view[1].img, img_right, npix, hipMemcpyHostToDevice, s));
+    if (init_disp_left)
+        HIP_TRY(&h->err, hipMemcpyAsync(h->init, init_disp_left, npix * 2, hipMemcpyHostToDevice, s));
     if (int rc = run_pipeline(h, disp_right != nullptr, s)) return rc;
     HIP_TRY(&h->err, hipMemcpyAsync(disp_left, h->view[0].out, npix * 4, hipMemcpyDeviceToHost, s));
     if (disp_right)
@@ -484,14 +559,13 @@ int sgm_match2(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
     if (!h->has_cam) return fail(&h->err, SGM_E_INVALID_ARG, "Match2 needs a handle set up by Init2");
     if (!img_left || !img_right || !depth_left)
         return fail(&h->err, SGM_E_INVALID_ARG, "image or output pointer is NULL");
-    if (init_disp_left)
-        return fail(&h->err, SGM_E_UNSUPPORTED,
-                    "initial disparities are used through the pyramid entry points");
-    if (roi_left || roi_right) return fail(&h->err, SGM_E_UNSUPPORTED, "ROIs are not built");
+    if (int rc = check_match_kind(h, init_disp_left, roi_left, roi_right)) return rc;
     const size_t npix = (size_t)h->W * h->H;
     hipStream_t s = h->stream;
     HIP_TRY(&h->err, hipMemcpyAsync(h->view[0].img, img_left, npix, hipMemcpyHostToDevice, s));
     HIP_TRY(&h->err, hipMemcpyAsync(h->view[1].img, img_right, npix, hipMemcpyHostToDevice, s));
+    if (init_disp_left)
+        HIP_TRY(&h->err, hipMemcpyAsync(h->init, init_disp_left, npix * 2, hipMemcpyHostToDevice, s));
     if (int rc = run_pipeline(h, false, s)) return rc;
     // the raw left map is free once the post stage has read it
     float* depth = h->view[0].raw;
@@ -527,6 +601,8 @@ int sgm_release(sgm_handle* h) {
     h->plan = StagePlan{};
     h->inited = false;
     h->has_cam = false;
+    h->window = false;
+    h->init = nullptr;
     return SGM_OK;
 }
 
@@ -1113,6 +1189,37 @@ int sgm_stage_aggregate(const uint8_t* d_cost, int32_t width, int32_t height,
     }
     j.p1 = p1; j.p2 = p2;
     return run_stage(j, nullptr, launch_aggregate, stream);
+}
+
+int sgm_stage_aggregate_wta_otf(const uint64_t* d_census_base, const uint64_t* d_census_match,
+                                int32_t width, int32_t height, int32_t min_disparity,
+                                int32_t disp_range, int32_t is_right, int32_t cs_type, int32_t p1,
+                                int32_t p2, float uniqueness, uint8_t* const d_paths[4],
+                                float* d_disp, void* stream) {
+    if (int rc = stage_check(width, height, min_disparity, disp_range)) return rc;
+    if (!d_census_base || !d_census_match || !d_paths || !d_disp)
+        return fail(nullptr, SGM_E_INVALID_ARG, "NULL pointer");
+    if (p1 < 0 || p2 < 0 || (int64_t)p1 + p2 > 65000)
+        return fail(nullptr, SGM_E_INVALID_ARG, "P1 and P2 must be >= 0 with P1 + P2 <= 65000");
+    SgmJob j = stage_job(width, height, min_disparity, disp_range);
+    j.census = const_cast<uint64_t*>(d_census_base);
+    j.census_other = d_census_match;
+    j.is_right = is_right ? 1 : 0;
+    j.cs_type = cs_type ? 1 : 0;
+    if (!wide_ok(j))
+        return fail(nullptr, SGM_E_UNSUPPORTED,
+                    "the on-the-fly cost takes D <= 256 and not the 5x5 right view");
+    for (int r : {(int)kUD, (int)kLR, (int)kRL}) {
+        if (!d_paths[r]) return fail(nullptr, SGM_E_INVALID_ARG, "NULL path pointer");
+        j.path[r] = d_paths[r];
+    }
+    j.disp = d_disp;
+    j.p1 = p1; j.p2 = p2;
+    j.uniqueness = uniqueness;
+    return run_stage(j, nullptr, [](const StagePlan& p, hipStream_t s) {
+        const hipError_t e = launch_wide3(p, s);
+        return e != hipSuccess ? e : launch_wide_wta(p, s);
+    }, stream);
 }
 
 int sgm_stage_wta(uint8_t* const d_paths[4], int32_t width, int32_t height,

@@ -94,6 +94,18 @@ inline int wide_v_items(const SgmJob& j, int kind) {
 // the 5x5 right view, whose skipped-pixel rule needs the stored cost.
 inline bool wide_ok(const SgmJob& j) { return j.D <= 256 && !(j.is_right && j.cs_type == 0); }
 
+// Caller buffers bound for one launch sequence in place of the plan's own:
+// the images of jobs 0 and 1 (the left and right view of a single-pair
+// plan) and the outputs of post 0.  With `on` = 0 the plan's buffers are
+// used; out_r may be null (no right output).  Passed to the census and post
+// kernels by value, so binding other buffers needs no plan upload.
+struct IoBind {
+    const uint8_t* img[2];
+    float* out_l;
+    float* out_r;
+    int32_t on;
+};
+
 // Device-side launch plan: jobs, per-stage prefix sums and totals.
 struct StagePlan {
     SgmJob* jobs = nullptr;       // device
@@ -114,6 +126,7 @@ struct StagePlan {
     int n_wh = 0, n_wv = 0, n_wf = 0;
     int dmax = 64;
     bool wide = false;            // matching uses the on-the-fly cost kernels
+    IoBind io{};                  // host-side: bound caller buffers, if any
 };
 
 // Scanline schedule of the persistent aggregation launches: packed items

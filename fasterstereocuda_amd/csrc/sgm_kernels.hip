@@ -66,11 +66,13 @@ __device__ __forceinline__ void census_column4(const uint8_t (*t)[72], int r0, i
 }
 
 __global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ jobs,
-                                                     const int* __restrict__ offs, int njobs) {
+                                                     const int* __restrict__ offs, int njobs,
+                                                     IoBind io) {
     __shared__ uint8_t tile[22][72];
     const int item = blockIdx.x;
     const int j = find_job(offs, njobs, item);
     const SgmJob& J = jobs[j];
+    const uint8_t* const img = (io.on && j < 2) ? io.img[j] : J.img;
     const int local = item - offs[j];
     const int tiles_x = (J.W + 63) / 64;
     const int x0 = (local % tiles_x) * 64, y0 = (local / tiles_x) * 16;
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ 
         const int r = idx / 72, c = idx % 72;
         const int yy = min(max(y0 - 3 + r, 0), J.H - 1);
         const int xx = min(max(x0 - 4 + c, 0), J.W - 1);
-        px[u] = (idx < kTileN) ? gld(J.img + (int64_t)yy * J.W + xx) : (uint8_t)0;
+        px[u] = (idx < kTileN) ? gld(img + (int64_t)yy * J.W + xx) : (uint8_t)0;
     }
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
@@ -589,11 +591,15 @@ __device__ __forceinline__ float median9(float a0, float a1, float a2, float b0,
 }
 
 __global__ __launch_bounds__(256) void post_kernel(const SgmPost* __restrict__ posts,
-                                                   const int* __restrict__ offs, int nposts) {
+                                                   const int* __restrict__ offs, int nposts,
+                                                   IoBind io) {
     __shared__ float t[18][18];
     const int item = blockIdx.x;
     const int j = find_job(offs, nposts, item);
     const SgmPost& P = posts[j];
+    const bool bound = io.on && j == 0;
+    float* const out_l = bound ? io.out_l : P.out_l;
+    float* const out_r = bound ? io.out_r : P.out_r;
     const int local = item - offs[j];
     const int tiles_x = (P.W + 15) / 16;
     const int x0 = (local % tiles_x) * 16, y0 = (local / tiles_x) * 16;
@@ -622,13 +628,13 @@ __global__ __launch_bounds__(256) void post_kernel(const SgmPost* __restrict__ p
     if (x >= P.W || y >= P.H) return;
     // the right output starts as the right map (LR invalidations follow in a
     // second launch) or, without the LR check, as all INVALID
-    if (P.out_r)
-        gst(P.out_r + (int64_t)y * P.W + x, P.do_lr ? raw_at(P.raw_r, P, x, y) : kInvalidDisp);
+    if (out_r)
+        gst(out_r + (int64_t)y * P.W + x, P.do_lr ? raw_at(P.raw_r, P, x, y) : kInvalidDisp);
     float v = t[ty + 1][tx + 1];
     if (P.do_median && x > 0 && y > 0)
         v = median9(t[ty][tx], t[ty][tx + 1], t[ty][tx + 2], t[ty + 1][tx], t[ty + 1][tx + 1],
                     t[ty + 1][tx + 2], t[ty + 2][tx], t[ty + 2][tx + 1], t[ty + 2][tx + 2]);
-    gst(P.out_l + (int64_t)y * P.W + x, v);
+    gst(out_l + (int64_t)y * P.W + x, v);
 }
 
 // Right-map invalidation of the LR check (cusgm_ca.cu:3894-3895): every left
@@ -636,11 +642,12 @@ __global__ __launch_bounds__(256) void post_kernel(const SgmPost* __restrict__ p
 // the right map.  All writers store the same value, so the result is fixed.
 __global__ __launch_bounds__(256) void right_invalidate_kernel(const SgmPost* __restrict__ posts,
                                                                const int* __restrict__ offs,
-                                                               int nposts) {
+                                                               int nposts, IoBind io) {
     const int item = blockIdx.x;
     const int j = find_job(offs, nposts, item);
     const SgmPost& P = posts[j];
-    if (!P.out_r || !P.do_lr) return;
+    float* const out_r = (io.on && j == 0) ? io.out_r : P.out_r;
+    if (!out_r || !P.do_lr) return;
     const int local = item - offs[j];
     const int tiles_x = (P.W + 15) / 16;
     const int x = (local % tiles_x) * 16 + (threadIdx.x & 15);
@@ -652,7 +659,7 @@ __global__ __launch_bounds__(256) void right_invalidate_kernel(const SgmPost* __
     const int xr = (xrf > 0) ? (int)((double)xrf + 0.5) : (int)((double)xrf - 0.5);
     if (xr < 0 || xr >= P.W) return;
     const float dr = raw_at(P.raw_r, P, xr, y);
-    if (fabsf(dl - dr) > P.lr_thr) gst(P.out_r + (int64_t)y * P.W + xr, kInvalidDisp);
+    if (fabsf(dl - dr) > P.lr_thr) gst(out_r + (int64_t)y * P.W + xr, kInvalidDisp);
 }
 
 // Depth from disparity (cusgm_util.cu:7-33): f1 = baseline * focus and
@@ -680,7 +687,7 @@ hipError_t launch_depth(const float* disp, float* depth, size_t n, double f1, do
 hipError_t launch_census(const StagePlan& p, hipStream_t s) {
     if (p.n_census <= 0) return hipSuccess;
     hipLaunchKernelGGL(census_kernel, dim3(p.n_census), dim3(256), 0, s, p.jobs, p.offs_census,
-                       p.njobs);
+                       p.njobs, p.io);
     return hipGetLastError();
 }
 
@@ -787,14 +794,14 @@ hipError_t launch_wta(const StagePlan& p, hipStream_t s) {
 hipError_t launch_post(const StagePlan& p, hipStream_t s) {
     if (p.n_post <= 0) return hipSuccess;
     hipLaunchKernelGGL(post_kernel, dim3(p.n_post), dim3(256), 0, s, p.posts, p.offs_post,
-                       p.nposts);
+                       p.nposts, p.io);
     return hipGetLastError();
 }
 
 hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s) {
     if (p.n_post <= 0) return hipSuccess;
     hipLaunchKernelGGL(right_invalidate_kernel, dim3(p.n_post), dim3(256), 0, s, p.posts,
-                       p.offs_post, p.nposts);
+                       p.offs_post, p.nposts, p.io);
     return hipGetLastError();
 }
 

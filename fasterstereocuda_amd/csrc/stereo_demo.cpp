@@ -5,6 +5,7 @@
 //               [--cs 0|1] [--p1 N] [--p2 N] [--uniq F] [--lr 0|1] [--lrthr F]
 //               [--median 0|1] [--repeat N] [--layers N]
 //               [--cam X0L,X0R,FOCUS,BASELINE] [--zrange MIN,MAX] [--peaks RATIO]
+//               [--default-opt 1] [--init INIT.i16]
 //
 // Without --layers: StereoCuda (single level, the options above).  With
 // --layers N: FasterStereoCuda (N-level pyramid over [MIN_D, MIN_D + RANGE),
@@ -14,6 +15,10 @@
 // interval from --zrange (the depth interval) and ignores MIN_D and RANGE.
 // --peaks turns on peak removal: StereoCuda with that ratio, FasterStereoCuda
 // with do_rm_peaks (its own ratio; any RATIO value).
+// --default-opt 1 starts StereoCuda from a default-constructed CuSGMOption
+// (the other options then set single fields) instead of the benchmark's.
+// --init passes an int16 W x H initial disparity map to StereoCuda::Match /
+// Match2 (RANGE 32: the window search of the reference's pyramid levels).
 // Inputs are tightly packed uint8 W x H; outputs tightly packed float32.
 // Prints the average Match time over --repeat calls after the first.
 #include <chrono>
@@ -57,14 +62,20 @@ int main(int argc, char** argv) {
     if (a < argc && std::strncmp(argv[a], "--", 2) != 0) out_right = argv[a++];
 
     CuSGMOption opt;
-    opt.cs_type = CuSGMOption::CS_9x7;
-    opt.p1 = 10;
-    opt.p2_init = 150;
-    opt.unique_threshold = 0.95f;
-    opt.do_lr_check = true;
-    opt.lr_threshold = 1.0f;
-    opt.do_median_filter = true;
-    opt.using_constant_p2 = true;
+    bool defaults = false;
+    for (int k = a; k + 1 < argc; k += 2)
+        if (std::strcmp(argv[k], "--default-opt") == 0) defaults = std::atoi(argv[k + 1]) != 0;
+    if (!defaults) {
+        opt.cs_type = CuSGMOption::CS_9x7;
+        opt.p1 = 10;
+        opt.p2_init = 150;
+        opt.unique_threshold = 0.95f;
+        opt.do_lr_check = true;
+        opt.lr_threshold = 1.0f;
+        opt.do_median_filter = true;
+        opt.using_constant_p2 = true;
+    }
+    const char* init_path = nullptr;
     int repeat = 1, layers = 0;
     bool depth = false;
     double cam[4] = {0, 0, 0, 0};      // x0_left, x0_right, focus, baseline
@@ -81,6 +92,8 @@ int main(int argc, char** argv) {
         else if (k == "--median") opt.do_median_filter = std::atoi(v) != 0;
         else if (k == "--repeat") repeat = std::atoi(v);
         else if (k == "--layers") layers = std::atoi(v);
+        else if (k == "--default-opt") continue;
+        else if (k == "--init") init_path = v;
         else if (k == "--peaks") {
             opt.do_remove_peaks = true;
             opt.peaks_ratio_threshold = (float)std::atof(v);
@@ -104,6 +117,17 @@ int main(int argc, char** argv) {
         return 2;
     }
     std::vector<float32> dl(npix), dr(npix);
+    std::vector<sint16> init;
+    if (init_path) {
+        std::vector<uint8> raw;
+        if (!read_file(init_path, raw, npix * 2)) {
+            std::fprintf(stderr, "cannot read the initial disparity map\n");
+            return 2;
+        }
+        init.resize(npix);
+        std::memcpy(init.data(), raw.data(), npix * 2);
+    }
+    sint16* const init_ptr = init_path ? init.data() : nullptr;
 
     if (layers > 0) {
         FasterStereoCuda fs;
@@ -154,9 +178,9 @@ int main(int argc, char** argv) {
     double total_ms = 0.0;
     for (int r = 0; r < repeat; ++r) {
         const auto t0 = std::chrono::steady_clock::now();
-        if (!(depth ? stereo.Match2(left.data(), right.data(), dl.data())
+        if (!(depth ? stereo.Match2(left.data(), right.data(), dl.data(), init_ptr)
                     : stereo.Match(left.data(), right.data(), dl.data(),
-                                   out_right ? dr.data() : nullptr))) {
+                                   out_right ? dr.data() : nullptr, init_ptr))) {
             std::fprintf(stderr, "Match failed: %s\n", stereo.LastError());
             return 1;
         }

@@ -16,6 +16,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
+from ._checks import device_tensor, host_input
 from ._lib import last_error, lib
 from .stereo import CamParam, CuSGMOption
 
@@ -63,15 +64,26 @@ class SgmPyramid:
     def device_bytes(self) -> int:
         return lib().sgm_pyramid_device_bytes(self._p)
 
+    def _inputs(self, left, right):
+        shape = (self.height, self.width)
+        return host_input(left, shape, np.uint8, "left"), host_input(right, shape, np.uint8, "right")
+
+    def _device(self, d_left, d_right, d_out, name):
+        import torch
+        shape = (self.height, self.width)
+        device_tensor(d_left, shape, torch.uint8, "d_left")
+        device_tensor(d_right, shape, torch.uint8, "d_right")
+        device_tensor(d_out, shape, torch.float32, name)
+
     def match(self, left: np.ndarray, right: np.ndarray) -> np.ndarray:
-        left = np.ascontiguousarray(left, np.uint8)
-        right = np.ascontiguousarray(right, np.uint8)
+        left, right = self._inputs(left, right)
         out = np.empty((self.height, self.width), np.float32)
         _lib.check(lib().sgm_pyramid_match(self._p, left.ctypes.data, right.ctypes.data,
                                            out.ctypes.data))
         return out
 
     def match_device(self, d_left, d_right, d_disp, stream=None) -> None:
+        self._device(d_left, d_right, d_disp, "d_disp")
         if stream is None:
             import torch
             stream = torch.cuda.current_stream().cuda_stream
@@ -80,14 +92,14 @@ class SgmPyramid:
 
     def match2(self, left: np.ndarray, right: np.ndarray) -> np.ndarray:
         """Depth map of a pair (pyramids made by from_depth)."""
-        left = np.ascontiguousarray(left, np.uint8)
-        right = np.ascontiguousarray(right, np.uint8)
+        left, right = self._inputs(left, right)
         out = np.empty((self.height, self.width), np.float32)
         _lib.check(lib().sgm_pyramid_match2(self._p, left.ctypes.data, right.ctypes.data,
                                             out.ctypes.data))
         return out
 
     def match2_device(self, d_left, d_right, d_depth, stream=None) -> None:
+        self._device(d_left, d_right, d_depth, "d_depth")
         if stream is None:
             import torch
             stream = torch.cuda.current_stream().cuda_stream
@@ -193,7 +205,7 @@ class FasterStereoCuda:
             return False
         try:
             disparity[...] = self._pyr.match(bytes_left, bytes_right)
-        except _lib.SgmError as e:
+        except (_lib.SgmError, ValueError) as e:
             self.last_error = str(e)
             return False
         return True
@@ -204,7 +216,7 @@ class FasterStereoCuda:
             return False
         try:
             depth[...] = self._pyr.match2(bytes_left, bytes_right)
-        except _lib.SgmError as e:
+        except (_lib.SgmError, ValueError) as e:
             self.last_error = str(e)
             return False
         return True

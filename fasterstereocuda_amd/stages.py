@@ -53,6 +53,26 @@ def aggregate(cost_vol: torch.Tensor, min_d: int, p1: int, p2: int):
     return paths
 
 
+def aggregate_wta_otf(census_base: torch.Tensor, census_match: torch.Tensor, min_d: int, D: int,
+                      is_right: bool, cs_type: int, p1: int, p2: int, uniqueness: float):
+    """The on-the-fly cost aggregation of one view: returns the top->bottom,
+    left->right and right->left volumes (index 2, bottom->top, is None: the
+    fused pass does not store it) and the raw WTA map."""
+    _dev(census_base, torch.int64)
+    _dev(census_match, torch.int64)
+    H, W = census_base.shape
+    if census_match.shape != (H, W):
+        raise ValueError("census maps differ in shape")
+    paths = [torch.zeros((H, W, D), dtype=torch.uint8, device=census_base.device)
+             if r != 2 else None for r in range(4)]
+    out = torch.empty((H, W), dtype=torch.float32, device=census_base.device)
+    ptrs = PP4(*[0 if p is None else p.data_ptr() for p in paths])
+    check(lib().sgm_stage_aggregate_wta_otf(census_base.data_ptr(), census_match.data_ptr(), W, H,
+                                            min_d, D, int(is_right), cs_type, p1, p2, uniqueness,
+                                            ptrs, out.data_ptr(), _stream()))
+    return paths, out
+
+
 def wta(paths, min_d: int, uniqueness: float) -> torch.Tensor:
     for p in paths:
         _dev(p, torch.uint8)

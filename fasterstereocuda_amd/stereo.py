@@ -13,6 +13,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
+from ._checks import device_tensor, host_array
 from ._lib import SgmCamera, SgmOption, SgmRoi, check, lib
 
 
@@ -135,6 +136,8 @@ class StereoCuda:
 
     def Init(self, width: int, height: int, min_disparity: int, disp_range: int,
              sgm_option: CuSGMOption, print_log: bool = False) -> bool:
+        """disp_range 64 * 2^n (n = 0..3), or 32 (LEVEL_RANGE) for a handle
+        that matches around an initial disparity map only."""
         opt = sgm_option.to_c()
         ok = self._ok(lib().sgm_init(self._h, width, height, min_disparity, disp_range,
                                      ctypes.byref(opt), int(print_log)))
@@ -145,15 +148,19 @@ class StereoCuda:
     def Match(self, img_left: np.ndarray, img_right: np.ndarray, disp_left: np.ndarray,
               disp_right: np.ndarray | None = None, init_disp_left=None,
               ste_roi_left=None, ste_roi_right=None) -> bool:
+        """init_disp_left (int16, the image shape) makes a disp_range 32
+        handle search 32 disparities around each pixel's value, as the
+        reference's finer pyramid levels do."""
         shape = (self.height, self.width)
-        for a, dt in ((img_left, np.uint8), (img_right, np.uint8), (disp_left, np.float32)):
-            if a.shape != shape or a.dtype != dt or not a.flags["C_CONTIGUOUS"]:
-                raise ValueError(f"expected C-contiguous {dt.__name__} array of shape {shape}")
-        if disp_right is not None and (disp_right.shape != shape or disp_right.dtype != np.float32
-                                       or not disp_right.flags["C_CONTIGUOUS"]):
-            raise ValueError("disp_right must be a C-contiguous float32 array of the image shape")
+        host_array(img_left, shape, np.uint8, "img_left")
+        host_array(img_right, shape, np.uint8, "img_right")
+        host_array(disp_left, shape, np.float32, "disp_left")
+        if disp_right is not None:
+            host_array(disp_right, shape, np.float32, "disp_right")
+        if init_disp_left is not None:
+            host_array(init_disp_left, shape, np.int16, "init_disp_left")
         ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
-        init = None if init_disp_left is None else init_disp_left.ctypes.data
+        init = ptr(init_disp_left)
         rl = None if ste_roi_left is None else ctypes.byref(ste_roi_left)
         rr = None if ste_roi_right is None else ctypes.byref(ste_roi_right)
         return self._ok(lib().sgm_match(self._h, ptr(img_left), ptr(img_right), ptr(disp_left),
@@ -175,9 +182,11 @@ class StereoCuda:
         """Match, then the float depth of the left map into depth_left
         (StereoCuda.h:69); invalid pixels stay -999999.0f."""
         shape = (self.height, self.width)
-        for a, dt in ((img_left, np.uint8), (img_right, np.uint8), (depth_left, np.float32)):
-            if a.shape != shape or a.dtype != dt or not a.flags["C_CONTIGUOUS"]:
-                raise ValueError(f"expected C-contiguous {dt.__name__} array of shape {shape}")
+        host_array(img_left, shape, np.uint8, "img_left")
+        host_array(img_right, shape, np.uint8, "img_right")
+        host_array(depth_left, shape, np.float32, "depth_left")
+        if init_disp_left is not None:
+            host_array(init_disp_left, shape, np.int16, "init_disp_left")
         init = None if init_disp_left is None else init_disp_left.ctypes.data
         rl = None if ste_roi_left is None else ctypes.byref(ste_roi_left)
         rr = None if ste_roi_right is None else ctypes.byref(ste_roi_right)
@@ -187,8 +196,14 @@ class StereoCuda:
     def match_device(self, d_left, d_right, d_disp_left, d_disp_right=None, stream=None) -> bool:
         """Pipeline on device tensors (uint8 / float32, C-contiguous), enqueued
         on `stream` (a hipStream_t as int, default: torch's current stream)."""
+        import torch
+        shape = (self.height, self.width)
+        device_tensor(d_left, shape, torch.uint8, "d_left")
+        device_tensor(d_right, shape, torch.uint8, "d_right")
+        device_tensor(d_disp_left, shape, torch.float32, "d_disp_left")
+        if d_disp_right is not None:
+            device_tensor(d_disp_right, shape, torch.float32, "d_disp_right")
         if stream is None:
-            import torch
             stream = torch.cuda.current_stream().cuda_stream
         return self._ok(lib().sgm_match_device(
             self._h, d_left.data_ptr(), d_right.data_ptr(), d_disp_left.data_ptr(),

@@ -307,6 +307,20 @@ int sgm_stage_cost(const uint64_t* d_census_base, const uint64_t* d_census_match
 int sgm_stage_aggregate(const uint8_t* d_cost, int32_t width, int32_t height,
                         int32_t min_disparity, int32_t disp_range, int32_t p1, int32_t p2,
                         uint8_t* const d_paths[4], void* stream);
+/* The aggregation the pipeline runs when the cost comes from the census
+ * words on the fly (sgm_set_cost_mode): cusgm_cc.cu:118-172 cost inside
+ * cusgm_ca.cu:1480-1888 recurrence, then the WTA (cusgm_ca.cu:3606-3809)
+ * fused with the bottom->top direction.  One view: the base census is this
+ * view's, is_right = 0 matches x - d (left view), 1 matches x + d.  Writes
+ * d_paths[0] (top->bottom), d_paths[1] (left->right), d_paths[3]
+ * (right->left) and the raw disparity map; d_paths[2] is not written (the
+ * fused pass keeps bottom->top in registers) and may be NULL.  D <= 256,
+ * not the 5x5 right view (SGM_E_UNSUPPORTED). */
+int sgm_stage_aggregate_wta_otf(const uint64_t* d_census_base, const uint64_t* d_census_match,
+                                int32_t width, int32_t height, int32_t min_disparity,
+                                int32_t disp_range, int32_t is_right, int32_t cs_type, int32_t p1,
+                                int32_t p2, float uniqueness, uint8_t* const d_paths[4],
+                                float* d_disp, void* stream);
 /* cusgm_ca.cu:3149-3232, 3606-3809. */
 int sgm_stage_wta(uint8_t* const d_paths[4], int32_t width, int32_t height,
                   int32_t min_disparity, int32_t disp_range, float uniqueness,

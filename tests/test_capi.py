@@ -132,6 +132,63 @@ def test_option_validation(fields, code):
     lib.sgm_destroy(h)
 
 
+def test_default_option_passes_validation():
+    """A default-constructed CuSGMOption (using_constant_p2 false,
+    sgm_types.h constructor) with only P1, P2 and the census type set is a
+    valid option: validation passes and the call reaches the device query
+    (which fails here, on a host without a GPU, with SGM_E_NO_DEVICE)."""
+    lib = _lib.lib()
+    h = lib.sgm_create()
+    opt = CuSGMOption(p1=10, p2_init=150, cs_type=1).to_c()
+    assert opt.using_constant_p2 == 0
+    rc = lib.sgm_init(h, 100, 50, 0, 64, ctypes.byref(opt), 0)
+    assert rc in (_lib.SGM_OK, _lib.SGM_E_NO_DEVICE, _lib.SGM_E_HIP), _lib.last_error(h)
+    # disp_range 32 (LEVEL_RANGE) is accepted too: matching around an initial map
+    rc = lib.sgm_init(h, 100, 50, 0, 32, ctypes.byref(opt), 0)
+    assert rc in (_lib.SGM_OK, _lib.SGM_E_NO_DEVICE, _lib.SGM_E_HIP), _lib.last_error(h)
+    lib.sgm_destroy(h)
+
+
+def test_python_bindings_check_buffers_before_any_call():
+    """Wrong shapes, dtypes, layouts or devices raise ValueError in the
+    bindings, before a pointer reaches the C ABI."""
+    import torch
+    from fasterstereocuda_amd.batch import SgmBatch
+    from fasterstereocuda_amd.pyramid import SgmPyramid
+    st = StereoCuda()
+    st.width, st.height = 8, 6
+    ok8, okf = np.zeros((6, 8), np.uint8), np.zeros((6, 8), np.float32)
+    bad = [
+        (np.zeros((6, 7), np.uint8), ok8, okf, None, None),
+        (ok8, ok8.astype(np.int8), okf, None, None),
+        (ok8, ok8, np.zeros((6, 8), np.float64), None, None),
+        (ok8, ok8, okf, np.zeros((5, 8), np.float32), None),
+        (ok8, ok8, okf, None, np.zeros((6, 8), np.int32)),
+        (ok8, ok8, np.zeros((8, 6), np.float32).T, None, None),
+    ]
+    for l, r, dl, dr, init in bad:
+        with pytest.raises(ValueError):
+            st.Match(l, r, dl, dr, init)
+    with pytest.raises(ValueError):
+        st.Match2(ok8, ok8, okf, np.zeros((6, 8), np.float32))
+    cpu8, cpuf = torch.zeros((6, 8), dtype=torch.uint8), torch.zeros((6, 8))
+    with pytest.raises(ValueError):
+        st.match_device(cpu8, cpu8, cpuf)
+    pyr = SgmPyramid.__new__(SgmPyramid)
+    pyr._p, pyr.width, pyr.height = None, 8, 6
+    with pytest.raises(ValueError):
+        pyr.match(np.zeros((6, 9), np.uint8), ok8)
+    with pytest.raises(ValueError):
+        pyr.match2(ok8, np.zeros((7, 8), np.uint8))
+    with pytest.raises(ValueError):
+        pyr.match_device(cpu8, cpu8, cpuf)
+    with pytest.raises(ValueError):
+        pyr.match2_device(cpu8, cpu8, cpuf)
+    with pytest.raises(ValueError):
+        SgmBatch([dict(left=cpu8, right=cpu8, disp_left=cpuf, disp_range=64)],
+                 CuSGMOption.benchmark())
+
+
 def test_peak_strips_setting():
     lib = _lib.lib()
     assert lib.sgm_get_peak_strips() == 1

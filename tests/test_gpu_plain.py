@@ -100,6 +100,32 @@ def test_stages_match_oracle(cuda, case):
     _eq_float(med_only.cpu().numpy(), oracle.median3x3(ref["wta_l"]), "median only")
 
 
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_on_the_fly_stages_match_oracle(cuda, case):
+    """The kernels the batched benchmark runs (cost from the census words
+    inside the aggregation, bottom->top fused with the WTA): their stored
+    top->bottom, left->right and right->left volumes and the raw WTA map of
+    both views, bit for bit against the oracle's stages (no median)."""
+    W, H, D, min_d, cs, p1, p2 = case
+    if D > 256:
+        pytest.skip("the on-the-fly cost takes D <= 256")
+    print(f"[otf stages] {case}", flush=True)
+    left, right = _pair(W, H, D, seed=7)
+    ref = oracle.stages(left, right, min_d, D, cs_type=cs, p1=p1, p2=p2, uniqueness=UNIQ,
+                        lr_threshold=LR_THR)
+    roi_x = max(0, min_d)
+    cl = torch.from_numpy(ref["census_l"].view(np.int64)).to(cuda)
+    cr = torch.from_numpy(ref["census_r"].view(np.int64)).to(cuda)
+    views = [("l", cl, cr, False)] + ([] if cs == 0 else [("r", cr, cl, True)])
+    for view, cb, cm, is_right in views:
+        paths, raw = stages.aggregate_wta_otf(cb, cm, min_d, D, is_right, cs, p1, p2, UNIQ)
+        for r in (0, 1, 3):
+            np.testing.assert_array_equal(paths[r].cpu().numpy()[:, roi_x:],
+                                          ref[f"paths_{view}"][r][:, roi_x:],
+                                          err_msg=f"path {stages.DIRS[r]} view {view}")
+        _eq_float(raw.cpu().numpy(), ref[f"wta_{view}"], f"raw wta {view}")
+
+
 @pytest.mark.parametrize("case", CASES[:6], ids=_ids)
 @pytest.mark.parametrize("lr,median", [(True, True), (False, True), (True, False)])
 def test_stereocuda_match_matches_oracle(cuda, case, lr, median):
@@ -131,19 +157,21 @@ def cost_mode():
 
 @pytest.mark.parametrize("mode", [2, 1], ids=["on-the-fly", "stored"])
 @pytest.mark.parametrize("case", CASES, ids=_ids)
-@pytest.mark.parametrize("lr", [True, False])
-def test_cost_modes_match_oracle(cuda, cost_mode, mode, case, lr):
+@pytest.mark.parametrize("lr,median", [(True, True), (False, True), (True, False), (False, False)])
+def test_cost_modes_match_oracle(cuda, cost_mode, mode, case, lr, median):
     """The on-the-fly cost kernels (census words read inside the aggregation,
     no cost volume) and the stored-cost kernels give the oracle's maps, for
-    every D, both census sizes, +-min_d, P1 > 127 and ragged sizes."""
+    every D, both census sizes, +-min_d, P1 > 127 and ragged sizes, with and
+    without the median (which could hide an isolated wrong disparity)."""
     W, H, D, min_d, cs, p1, p2 = case
-    print(f"[cost mode {mode}] {case} lr={lr}", flush=True)
+    print(f"[cost mode {mode}] {case} lr={lr} median={median}", flush=True)
     cost_mode(mode)
     left, right = _pair(W, H, D, seed=5)
     exp_l, exp_r = oracle.match(left, right, min_d, D, cs_type=cs, p1=p1, p2=p2, uniqueness=UNIQ,
-                                do_lr_check=lr, lr_threshold=LR_THR, do_median=True)
+                                do_lr_check=lr, lr_threshold=LR_THR, do_median=median)
     st = StereoCuda()
-    opt = CuSGMOption.benchmark(cs_type=cs, p1=p1, p2_init=p2, do_lr_check=lr)
+    opt = CuSGMOption.benchmark(cs_type=cs, p1=p1, p2_init=p2, do_lr_check=lr,
+                                do_median_filter=median)
     assert st.Init(W, H, min_d, D, opt), st.last_error
     dl = np.empty((H, W), np.float32)
     dr = np.empty((H, W), np.float32)
@@ -171,6 +199,35 @@ def test_driving_full_size(cuda):
     assert st.match_device(Ld, Rd, od)
     torch.cuda.synchronize()
     _eq_float(od.cpu().numpy(), exp_l, "device entry point")
+
+
+@pytest.mark.parametrize("cs,lr", [(1, True), (0, True), (1, False)])
+def test_match_device_binds_caller_buffers(cuda, cs, lr):
+    """match_device reads the caller's images and writes the caller's maps in
+    place (no staging copies): alternating buffer sets, with and without the
+    right map and peak removal, interleaved with host Match calls."""
+    W, H, D = 220, 90, 64
+    opt = CuSGMOption.benchmark(cs_type=cs, do_lr_check=lr, do_remove_peaks=True,
+                                peaks_ratio_threshold=0.002)
+    st = StereoCuda()
+    assert st.Init(W, H, 0, D, opt), st.last_error
+    pairs = [_pair(W, H, D, seed=s) for s in (21, 22)]
+    exps = [oracle.match(l, r, 0, D, cs_type=cs, do_lr_check=lr, peaks_ratio=0.002) for l, r in pairs]
+    bufs = [(torch.from_numpy(l).to(cuda), torch.from_numpy(r).to(cuda),
+             torch.full((H, W), 7.0, device=cuda), torch.full((H, W), 7.0, device=cuda))
+            for l, r in pairs]
+    for rep in range(2):
+        for k, ((L, R, ol, orr), (el, er)) in enumerate(zip(bufs, exps)):
+            print(f"[match_device] cs={cs} lr={lr} rep={rep} pair={k}", flush=True)
+            want_right = (rep + k) % 2 == 0
+            assert st.match_device(L, R, ol, orr if want_right else None), st.last_error
+            torch.cuda.synchronize()
+            _eq_float(ol.cpu().numpy(), el, "device left")
+            if want_right:
+                _eq_float(orr.cpu().numpy(), er, "device right")
+            dl = np.empty((H, W), np.float32)
+            assert st.Match(pairs[k][0], pairs[k][1], dl), st.last_error
+            _eq_float(dl, el, "host left after a device match")
 
 
 def test_set_min_disparity_rebuilds_the_plan(cuda):
