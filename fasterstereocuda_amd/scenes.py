@@ -40,7 +40,9 @@ def _rng(workload: int, index: int) -> np.random.Generator:
 
 def value_noise(rng: np.random.Generator, H: int, W: int, cells=(16, 8, 4, 2)) -> np.ndarray:
     """Sum of octaves of bilinearly upsampled U[0,1) noise, amplitude halving
-    per octave, rescaled to [0, 255] (float32)."""
+    per octave, rescaled to [0, 255] (float32).  The interpolation runs along
+    x on the coarse rows first, then along y: the same products and sums per
+    pixel as the direct bilinear formula, in the same order."""
     acc = np.zeros((H, W), np.float64)
     amp = 1.0
     for s in cells:
@@ -52,11 +54,8 @@ def value_noise(rng: np.random.Generator, H: int, W: int, cells=(16, 8, 4, 2)) -
         x0 = np.floor(fx).astype(np.int64)
         wy = (fy - y0)[:, None]
         wx = (fx - x0)[None, :]
-        a = g[np.ix_(y0, x0)]
-        b = g[np.ix_(y0, x0 + 1)]
-        c = g[np.ix_(y0 + 1, x0)]
-        d = g[np.ix_(y0 + 1, x0 + 1)]
-        acc += amp * ((a * (1 - wx) + b * wx) * (1 - wy) + (c * (1 - wx) + d * wx) * wy)
+        rows = g[:, x0] * (1 - wx) + g[:, x0 + 1] * wx          # (gh, W)
+        acc += amp * (rows[y0] * (1 - wy) + rows[y0 + 1] * wy)
         amp *= 0.5
     lo, hi = acc.min(), acc.max()
     return ((acc - lo) * (255.0 / max(hi - lo, 1e-12))).astype(np.float32)
@@ -85,21 +84,24 @@ def _warp_right(rng: np.random.Generator, left: np.ndarray, disp: np.ndarray,
 
 def _planes(rng: np.random.Generator, H: int, W: int, n: int, dmin: float, dmax: float,
             slope: float) -> np.ndarray:
-    """Piecewise-planar disparity: n planes on a Voronoi partition."""
+    """Piecewise-planar disparity: n planes on a Voronoi partition (each
+    pixel takes the plane of its nearest centre, the lower index on a tie).
+    Distances and plane values are float64, evaluated in row blocks."""
     cy = rng.random(n) * H
     cx = rng.random(n) * W
     base = dmin + rng.random(n) * (dmax - dmin)
     gx = (rng.random(n) * 2 - 1) * slope
     gy = (rng.random(n) * 2 - 1) * slope
-    ys, xs = np.mgrid[0:H, 0:W].astype(np.float32)
-    best = np.full((H, W), np.inf, np.float32)
-    disp = np.zeros((H, W), np.float32)
-    for k in range(n):
-        dist = (ys - cy[k]) ** 2 + (xs - cx[k]) ** 2
-        sel = dist < best
-        best = np.where(sel, dist, best)
-        plane = base[k] + gx[k] * (xs - cx[k]) + gy[k] * (ys - cy[k])
-        disp = np.where(sel, plane, disp)
+    xs = np.arange(W, dtype=np.float64)
+    dx2 = (xs[None, :] - cx[:, None]) ** 2                       # (n, W)
+    disp = np.empty((H, W), np.float64)
+    rows = max(1, (1 << 23) // max(1, n * W))
+    for y0 in range(0, H, rows):
+        ys = np.arange(y0, min(H, y0 + rows), dtype=np.float64)
+        dy2 = (ys[None, :] - cy[:, None]) ** 2                   # (n, r)
+        k = np.argmin(dy2[:, :, None] + dx2[:, None, :], axis=0)  # (r, W), first minimum
+        disp[y0:y0 + len(ys)] = (base[k] + gx[k] * (xs[None, :] - cx[k])
+                                 + gy[k] * (ys[:, None] - cy[k]))
     return np.clip(disp, dmin, dmax).astype(np.float32)
 
 
