@@ -30,6 +30,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads stay on neighbouring cores; set before any
+# OpenMP runtime starts (the thread count is the environment's own)
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per cell and view of aggregation + WTA: 4 direction reads
@@ -50,7 +53,13 @@ def parse():
     ap.add_argument("--range", dest="disp_range", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain path")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-runs", type=int, default=5,
+                    help="timed CPU baseline runs (median reported), after one warm-up")
+    ap.add_argument("--cpu-pairs", type=int, default=16,
+                    help="pairs of the workload per CPU baseline run")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the pyramid, uneven-batch and differentiable workloads in the "
+                         "default line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a float32 negate of 256 MiB (counter calibration)")
     ap.add_argument("--workload", default="driving",
@@ -172,31 +181,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle                                    # CPU baseline leg only
-        oracle.lib()
-        oracle.match(pairs_np[0].left, pairs_np[0].right, 0, D)     # warm-up
-        done = 0
-        tc0 = time.perf_counter()
-        exact = None
-        while True:
-            p = pairs_np[done % NP]
-            dl, _ = oracle.match(p.left, p.right, 0, D)
-            if done == 0:
-                exact = bool((dl.view(np.uint32) == batched0.view(np.uint32)).all())
-            done += 1
-            if time.perf_counter() - tc0 >= args.cpu_seconds or done >= 4 * NP:
-                break
-        tc = time.perf_counter() - tc0
-        cpu = {
-            "value": done * cells_per_pair / tc,
-            "unit": "W*H*D/s",
-            "cores": oracle.num_threads(),
-            "kind": "port",
-            "sample": f"{done} pairs of the same workload, sequential, {tc:.1f} s",
-            "gpu_output_bit_exact_vs_oracle": exact,
-            "cpu_model": cpu_model(),
-        }
+        cpu = cpu_baseline(pairs_np, D, batched0, args.cpu_runs, args.cpu_pairs)
 
     # host-pointer path (the reference's calling pattern: copies over PCIe
     # inside every Match), one pair at a time, both reference image sizes
@@ -242,18 +227,102 @@ def main():
         "plain_bit_exact_vs_batched": same_as_plain,
         "stage_ms": stage_avg,
     }
+    if not args.no_extra:
+        # the other measured workloads, each on its own data, after the headline
+        del pairs, batch
+        torch.cuda.empty_cache()
+        for key, fn in (("diffsgm_batch8_1242x375_d128", diffsgm),
+                        ("pyramid_building_4800x3409_d256", building),
+                        ("uneven_batch64", uneven)):
+            line[key] = fn(args, quick=True)
+            torch.cuda.empty_cache()
     print(json.dumps(line), flush=True)
 
 
-def cpu_model() -> str:
+def cpu_baseline(pairs_np, D, gpu_first, runs, per_run):
+    """The OpenMP oracle on the host cores: one warm-up, then the median of
+    `runs` runs, each matching `per_run` pairs of the workload in sequence."""
+    import numpy as np
+    import statistics
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle                                    # CPU baseline leg only
+    oracle.lib()
+    H, W = pairs_np[0].left.shape
+    first, _ = oracle.match(pairs_np[0].left, pairs_np[0].right, 0, D)      # warm-up
+    exact = bool((first.view(np.uint32) == gpu_first.view(np.uint32)).all())
+    rates = []
+    for r in range(runs):
+        t0 = time.perf_counter()
+        for k in range(per_run):
+            p = pairs_np[(r * per_run + k) % len(pairs_np)]
+            oracle.match(p.left, p.right, 0, D)
+        rates.append(per_run * W * H * D / (time.perf_counter() - t0))
+    phys, logical = host_cores()
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {
+        "value": statistics.median(rates),
+        "unit": "W*H*D/s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": f"median of {runs} runs after 1 warm-up; a run = {per_run} pairs of the "
+                  f"workload ({W}x{H}, D={D}) matched in sequence, each pair parallel over "
+                  "rows / scanlines",
+        "runs": rates,
+        "gpu_output_bit_exact_vs_oracle": exact,
+        "cpu_model": cpu_model(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+        "cpus_usable": affinity,
+        "machine_physical_cores": phys,
+        "machine_logical_cpus": logical,
+    }
+
+
+def host_cores():
+    """(physical cores, logical CPUs) of the machine from /proc/cpuinfo."""
+    phys, logical = set(), 0
+    cur = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    if "core id" in cur:
+                        phys.add((cur.get("physical id"), cur["core id"]))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "processor":
+                    logical += 1
+                cur[k] = v.strip()
+        if "core id" in cur:
+            phys.add((cur.get("physical id"), cur["core id"]))
     except OSError:
-        pass
-    return "unknown"
+        return None, None
+    return (len(phys) or None), (logical or None)
+
+
+def cpu_model() -> str:
+    """The CPU's model name with its family / model / stepping numbers (some
+    virtual machines report only a short marketing name)."""
+    info = {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in ("model name", "cpu family", "model", "stepping", "vendor_id") and k not in info:
+                    info[k] = v.strip()
+                if not line.strip() and info:
+                    break
+    except OSError:
+        return "unknown"
+    name = info.get("model name", "unknown")
+    return (f"{name} ({info.get('vendor_id', '?')} family {info.get('cpu family', '?')} "
+            f"model {info.get('model', '?')} stepping {info.get('stepping', '?')})")
 
 
 def host_match_rates(opt, reps: int = 20):
@@ -332,7 +401,7 @@ def _common(args, metric, value, unit, ms, hib, config, **extra):
     return line
 
 
-def building(args):
+def building(args, quick=False):
     import numpy as np
     import torch
     from fasterstereocuda_amd import scenes
@@ -347,14 +416,15 @@ def building(args):
     out = torch.empty((H, W), dtype=torch.float32, device=dev)
     opt = CuSGMOption.benchmark()
     pyr = SgmPyramid(W, H, 0, D, 3, 16, opt)
-    for _ in range(args.warmup):
+    steps = min(args.steps, 10) if quick else args.steps
+    for _ in range(max(1, args.warmup)):
         pyr.match_device(L, R, out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         pyr.match_device(L, R, out)
     torch.cuda.synchronize()
-    pyr_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    pyr_ms = (time.perf_counter() - t0) * 1e3 / steps
     pyr_out = out.cpu().numpy()
     # dense single level, D = 256 (both views, LR check): streams ~4.2 GB per volume
     st = StereoCuda()
@@ -362,7 +432,7 @@ def building(args):
     dense = torch.empty_like(out)
     st.match_device(L, R, dense)
     torch.cuda.synchronize()
-    dsteps = max(1, args.steps // 4)
+    dsteps = 2 if quick else max(1, args.steps // 4)
     t0 = time.perf_counter()
     for _ in range(dsteps):
         st.match_device(L, R, dense)
@@ -371,6 +441,12 @@ def building(args):
     st.Release()
     cells = W * H * D
     valid = pyr_out != -999999.0
+    if quick:
+        return {"workload": "building 4800x3409, D=256, 3-level pyramid, window 16",
+                "ms_per_pair": pyr_ms, "value": cells / (pyr_ms * 1e-3), "unit": "W*H*D/s (nominal D)",
+                "steps": steps, "valid_fraction": float(valid.mean()),
+                "dense_single_level": {"ms_per_pair": dense_ms, "value": cells / (dense_ms * 1e-3),
+                                       "unit": "W*H*D/s", "steps": dsteps}}
     _emit(_common(args, "W*H*D/s (nominal D) large-image pyramid", cells / (pyr_ms * 1e-3), "W*H*D/s",
                   pyr_ms, True,
                   {"workload": "building 4800x3409, D=256, 3-level pyramid, window 16, 9x7 census, "
@@ -382,7 +458,7 @@ def building(args):
                   pyramid_valid_fraction=float(valid.mean())))
 
 
-def uneven(args):
+def uneven(args, quick=False):
     import numpy as np
     import torch
     from fasterstereocuda_amd import scenes
@@ -400,25 +476,26 @@ def uneven(args):
                       "min_disparity": 0, "disp_range": d})
         cells += w * h * d
     batch = SgmBatch(pairs, opt)
+    steps = min(args.steps, 10) if quick else args.steps
     # grid-schedule eager time, then the autotuned schedule run as one graph
     for _ in range(max(1, args.warmup)):
         batch.run()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         batch.run()
     torch.cuda.synchronize()
-    grid_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    grid_ms = (time.perf_counter() - t0) * 1e3 / steps
     policy, waves, tuned_ms = batch.autotune(reps=3, force=True)
     batch.use_graph(True)
-    for _ in range(args.warmup):
+    for _ in range(max(1, args.warmup)):
         batch.run()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         batch.run()
     torch.cuda.synchronize()
-    b_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    b_ms = (time.perf_counter() - t0) * 1e3 / steps
     # plain path: one handle per (W, H, D), one Match per pair
     handles = {}
     outs = []
@@ -436,7 +513,7 @@ def uneven(args):
             handles[(w, h, p["disp_range"])].match_device(p["left"], p["right"], o)
     plain_pass()
     torch.cuda.synchronize()
-    psteps = max(1, args.steps // 2)
+    psteps = max(1, steps // 2)
     t0 = time.perf_counter()
     for _ in range(psteps):
         plain_pass()
@@ -444,6 +521,17 @@ def uneven(args):
     p_ms = (time.perf_counter() - t0) * 1e3 / psteps
     same = all(bool((o.cpu().numpy().view(np.uint32) == p["disp_left"].cpu().numpy().view(np.uint32)).all())
                for o, p in zip(outs, pairs))
+    sched = {"policy": ["grid", "job-major", "longest-first"][policy], "waves": waves,
+             "autotune_ms": tuned_ms, "grid_eager_ms": grid_ms, "graph": True}
+    for st in handles.values():
+        st.Release()
+    if quick:
+        return {"workload": "64 pairs, sizes from {450x375, 640x480, 1242x375, 1280x720, "
+                            "1920x1080}, D in {64, 128}",
+                "value": cells / (b_ms * 1e-3), "unit": "W*H*D/s", "ms_per_batch": b_ms,
+                "plain_value": cells / (p_ms * 1e-3), "plain_ms_per_batch": p_ms,
+                "batched_over_plain": p_ms / b_ms, "batched_bit_exact_vs_plain": same,
+                "steps": steps, "schedule": sched}
     _emit(_common(args, "W*H*D/s summed over an uneven batch (batched vs plain)", cells / (b_ms * 1e-3),
                   "W*H*D/s", b_ms, True,
                   {"workload": "64 pairs, sizes from {450x375, 640x480, 1242x375, 1280x720, 1920x1080}, "
@@ -451,12 +539,10 @@ def uneven(args):
                    "pairs": 64, "path": "batched"},
                   dtype="u8", plain_value=cells / (p_ms * 1e-3), plain_ms=p_ms,
                   batched_over_plain=p_ms / b_ms, batched_bit_exact_vs_plain=same,
-                  schedule={"policy": ["grid", "job-major", "longest-first"][policy], "waves": waves,
-                            "autotune_ms": tuned_ms, "grid_eager_ms": grid_ms, "graph": True},
-                  device_bytes=batch.device_bytes))
+                  schedule=sched, device_bytes=batch.device_bytes))
 
 
-def diffsgm(args):
+def diffsgm(args, quick=False):
     import numpy as np
     import torch
     from fasterstereocuda_amd import scenes
@@ -482,25 +568,45 @@ def diffsgm(args):
         loss.backward()
         return loss
 
-    for _ in range(args.warmup):
+    steps = min(args.steps, 10) if quick else args.steps
+    for _ in range(max(1, args.warmup)):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         loss = step()
     torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms = (time.perf_counter() - t0) * 1e3 / steps
     cells = B * H * W * D
     bytes_alg = 50.0 * cells                      # ~24 B fwd + ~26 B bwd per cell
+    roof = {"bound": "hbm", "achieved": bytes_alg / (ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": bytes_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": _profile_traffic("diffsgm_traffic.json", (B, W, H, D)),
+            "algorithmic_bytes_per_step": bytes_alg}
+    if quick:
+        return {"workload": "differentiable FP32 SGM, batch 8, 1242x375, D=128, per-pixel P1/P2, "
+                            "soft-argmin T=0.1, L1 loss",
+                "ms_per_pair": ms / B, "ms_per_step": ms, "unit": "ms/pair", "steps": steps,
+                "loss": float(loss), "roofline": roof}
     _emit(_common(args, "diff-SGM fwd+bwd ms/pair", ms / B, "ms/pair", ms, False,
                   {"workload": "differentiable FP32 SGM, batch 8, 1242x375, D=128, per-pixel P1/P2, "
                                "soft-argmin T=0.1, L1 loss", "batch": B, "width": W, "height": H,
                    "disp_range": D},
-                  dtype="fp32", loss=float(loss),
-                  roofline={"bound": "hbm", "achieved": bytes_alg / (ms * 1e-3) / 1e9,
-                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": bytes_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                            "traffic": None, "algorithmic_bytes_per_step": bytes_alg}))
+                  dtype="fp32", loss=float(loss), roofline=roof))
+
+
+def _profile_traffic(name, key):
+    """Measured HBM bytes of a workload's step from a committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary, if one matches the workload."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if tuple(t.get("key", ())) != tuple(key):
+        return None
+    return t.get("bytes_per_step")
 
 
 def run_other(args):

@@ -4,6 +4,8 @@
   fasterstereocuda_amd/libStereoCuda.so        StereoCuda drop-in class over the C ABI
   fasterstereocuda_amd/libFasterStereoCuda.so  FasterStereoCuda drop-in class over the C ABI
   fasterstereocuda_amd/stereo_demo             C++ program that drives both classes
+  fasterstereocuda_amd/stereo_console          the reference console's command line
+                                               (option.xml, PNG / PGM in, PFM / PNG out)
 
 Run as ``python -m fasterstereocuda_amd.build``.
 """
@@ -36,6 +38,7 @@ LIB = os.path.join(PKG, "libfsgm.so")
 CXX_LIB = os.path.join(PKG, "libStereoCuda.so")
 FS_LIB = os.path.join(PKG, "libFasterStereoCuda.so")
 DEMO = os.path.join(PKG, "stereo_demo")
+CONSOLE = os.path.join(PKG, "stereo_console")
 
 
 def _hipcc() -> str:
@@ -78,6 +81,11 @@ def build_cxx(verbose: bool = False) -> None:
     _run([cxx, "-O2", "-std=c++17", "-I", INCLUDE, os.path.join(CSRC, "stereo_demo.cpp"),
           "-L", PKG, "-lStereoCuda", "-lFasterStereoCuda", "-lfsgm", rpath, "-o", tmp], verbose)
     os.replace(tmp, DEMO)
+    tmp = CONSOLE + ".tmp"
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", INCLUDE,
+          *[os.path.join(CSRC, f) for f in ("stereo_console.cpp", "option_xml.cpp", "image_io.cpp")],
+          "-L", PKG, "-lFasterStereoCuda", "-lfsgm", "-lz", rpath, "-o", tmp], verbose)
+    os.replace(tmp, CONSOLE)
 
 
 def build_all(verbose: bool = False) -> None:
@@ -87,4 +95,4 @@ def build_all(verbose: bool = False) -> None:
 
 if __name__ == "__main__":
     build_all(verbose="-v" in sys.argv)
-    print("built", LIB, CXX_LIB, FS_LIB, DEMO)
+    print("built", LIB, CXX_LIB, FS_LIB, DEMO, CONSOLE)

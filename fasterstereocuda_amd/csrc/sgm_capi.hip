@@ -1216,6 +1216,10 @@ int sgm_stage_aggregate_wta_otf(const uint64_t* d_census_base, const uint64_t* d
     j.disp = d_disp;
     j.p1 = p1; j.p2 = p2;
     j.uniqueness = uniqueness;
+    // the fused WTA writes the columns from roi_x on; the ones left of them
+    // are INVALID, as the WTA stage writes them
+    HIP_TRY(nullptr, launch_invalid_columns(d_disp, width, height, j.roi_x,
+                                            static_cast<hipStream_t>(stream)));
     return run_stage(j, nullptr, [](const StagePlan& p, hipStream_t s) {
         const hipError_t e = launch_wide3(p, s);
         return e != hipSuccess ? e : launch_wide_wta(p, s);

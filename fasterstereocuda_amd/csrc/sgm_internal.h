@@ -173,6 +173,8 @@ hipError_t launch_remove_peaks(float* disp, int* scratch, int W, int H, int stri
 int peaks_threshold(int W, int strip_rows, float ratio);
 int peak_strips();   // process-wide strip count (sgm_set_peak_strips)
 // depth = (float)(f1 / ((double)disp + f2)), invalid kept (n pixels)
+// columns [0, ncols) of a W x H float map set to INVALID
+hipError_t launch_invalid_columns(float* disp, int W, int H, int ncols, hipStream_t s);
 hipError_t launch_depth(const float* disp, float* depth, size_t n, double f1, double f2,
                         hipStream_t s);
 

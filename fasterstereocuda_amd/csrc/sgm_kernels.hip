@@ -674,7 +674,27 @@ __global__ __launch_bounds__(256) void depth_kernel(const float* __restrict__ di
     }
 }
 
+// Columns [0, ncols) of every row set to INVALID: the part of a raw map left
+// of the matched region, which the fused on-the-fly WTA does not write.
+__global__ __launch_bounds__(256) void invalid_columns_kernel(float* __restrict__ disp, int W,
+                                                              int H, int ncols) {
+    const int64_t n = (int64_t)ncols * H;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t y = i / ncols, x = i - y * ncols;
+        disp[y * W + x] = kInvalidDisp;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_invalid_columns(float* disp, int W, int H, int ncols, hipStream_t s) {
+    ncols = std::min(ncols, W);
+    if (ncols <= 0 || H <= 0) return hipSuccess;
+    const int64_t n = (int64_t)ncols * H;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65535);
+    hipLaunchKernelGGL(invalid_columns_kernel, dim3(blocks), dim3(256), 0, s, disp, W, H, ncols);
+    return hipGetLastError();
+}
 
 hipError_t launch_depth(const float* disp, float* depth, size_t n, double f1, double f2,
                         hipStream_t s) {

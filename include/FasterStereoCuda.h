@@ -24,8 +24,8 @@
 
 class FASTERSTEREOCUDA_DLL FasterStereoCuda {
 public:
-    /* rectified-pair geometry (reference :28-37); used only by the depth
-     * interface, which this build does not provide */
+    /* rectified-pair geometry (reference :28-37), read by the depth
+     * interface (Init2 / Match2) */
     typedef struct EpipolarStereoParam {
         double x0_left, y0_left;
         double x0_right, y0_right;
@@ -42,8 +42,12 @@ public:
         int max_disp;
         int num_layers;
         bool do_lr_check;
-        bool do_rm_peaks;    /* must be false: peak removal is not built */
-        bool do_smooth;      /* must be false: post filters are not built */
+        bool do_rm_peaks;    /* peak removal on the GPU after the coarsest level's
+                                filters, with the reference's ratio per level
+                                count; segments are whole-image connected
+                                components (no per-thread strips) */
+        bool do_smooth;      /* must be false: the Gauss post filter is not built
+                                (Init returns false) */
         StereoOption1()
             : width(0), height(0), min_disp(0), max_disp(0), num_layers(1), do_lr_check(true),
               do_rm_peaks(true), do_smooth(false) {}
@@ -78,7 +82,9 @@ public:
      * LR check at the coarsest level, 3x3 median, dilation of the coarsest
      * level, 32-disparity windows on the finer levels. */
     bool Init(const ste_opt1_t& option) const;
-    /* Depth output is not part of this build: returns false. */
+    /* Depth form (reference FasterStereoCuda.cpp:297-406): the disparity
+     * interval comes from [min_depth, max_depth] and the camera in
+     * option.epi; otherwise as Init. */
     bool Init2(const ste_opt2_t& option) const;
     void Release() const;
 
@@ -86,7 +92,9 @@ public:
      * into a float disparity map; invalid pixels hold -999999.0f. */
     bool Match(const unsigned char* bytes_left, const unsigned char* bytes_right,
                float* disparity_data) const;
-    /* Depth output is not part of this build: returns false. */
+    /* Match, then depth = baseline * focus / (disp + x0_right - x0_left) of
+     * the final map on the GPU (float64 arithmetic); invalid pixels hold
+     * -999999.0f. */
     bool Match2(const unsigned char* bytes_left, const unsigned char* bytes_right,
                 float* depth_data) const;
 

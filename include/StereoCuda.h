@@ -28,18 +28,26 @@ public:
     StereoCuda& operator=(const StereoCuda&) = delete;
 
     /* width x height pair, disparities [min_disparity, min_disparity + disp_range);
-     * disp_range = 64 * 2^n (n = 0..3). */
+     * disp_range = 64 * 2^n (n = 0..3), or get_level_range() (32) for an
+     * object that matches only around an initial disparity map. */
     bool Init(sint32 width, sint32 height, sint32 min_disparity, sint32 disp_range,
               CuSGMOption sgm_option, bool print_log = false) const;
 
     /* Blocking match of a rectified uint8 pair into float disparity maps
-     * (INVALID_VALUE marks invalid pixels).  disp_right may be nullptr. */
+     * (INVALID_VALUE marks invalid pixels).  disp_right may be nullptr.
+     * init_disp_left (sint16, INVALID = get_invad_short()) is taken by an
+     * object initialised with disp_range 32, do_lr_check false and
+     * using_constant_p2: it matches in a 32-disparity window starting at each
+     * pixel's initial disparity, as the coarse-to-fine levels do.  The ROIs
+     * must be nullptr: ROI-restricted matching is not built (returns false). */
     bool Match(uint8* img_left, uint8* img_right, float32* disp_left,
                float32* disp_right = nullptr, sint16* init_disp_left = nullptr,
                StereoROI_T* ste_roi_left = nullptr, StereoROI_T* ste_roi_right = nullptr) const;
 
-    /* Depth output (reference StereoCuda.h:57,69) is not part of this build:
-     * both return false. */
+    /* Depth output (reference StereoCuda.h:57,69): Init as above plus the
+     * camera; Match2 runs Match and converts the final left map on the GPU,
+     * depth = baseline * focus / (disp + x0_right - x0_left) in float64,
+     * with invalid pixels passed through as get_invad_float(). */
     bool Init2(sint32 width, sint32 height, sint32 min_disparity, sint32 disp_range,
                CuSGMOption sgm_option, CamParam_T cam_param, bool print_log = false) const;
     bool Match2(uint8* img_left, uint8* img_right, float32* depth_left,

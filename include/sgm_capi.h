@@ -69,20 +69,30 @@ void sgm_destroy(sgm_handle* h);
 
 /* Replaces StereoCuda::Init (StereoCuda.h:31, StereoCudaImpl.cpp:128-261).
  * Arguments are validated before the device is touched.  disp_range must be
- * 64 * 2^n with n in 0..3; P1 and P2 must be >= 0 with P1 + P2 <= 65000. */
+ * 64 * 2^n with n in 0..3, or 32 (LEVEL_RANGE) for a handle that matches only
+ * around an initial disparity map; P1 and P2 must be >= 0 with
+ * P1 + P2 <= 65000.  using_constant_p2 is read only with an initial map, as in
+ * the reference (cusgm_ca.cu:4005-4008). */
 int sgm_init(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity,
              int32_t disp_range, const sgm_option* opt, int32_t print_log);
 
 /* Replaces StereoCuda::Match (StereoCuda.h:44, StereoCudaImpl.cpp:507-546):
- * host pointers, blocking.  disp_right may be NULL.  init_disp_left and the
- * ROIs must be NULL (initial disparities are reached through the pyramid
- * entry points; ROIs are not built). */
+ * host pointers, blocking.  disp_right may be NULL.
+ *   - init_disp_left NULL: the plain pass (disp_range 64 * 2^n handles).
+ *   - init_disp_left set: a disp_range 32 handle matches in a 32-disparity
+ *     window around each pixel's initial disparity (window cost, window
+ *     aggregation, window WTA, median; StereoCudaImpl.cpp:363-446,
+ *     cusgm_ca.cu:4052-4077).  That needs do_lr_check = 0, using_constant_p2
+ *     and min_disparity <= 0; disp_right, if given, is all INVALID.
+ * The ROIs must be NULL (SGM_E_UNSUPPORTED): ROI-restricted matching is not
+ * built. */
 int sgm_match(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
               float* disp_left, float* disp_right, const int16_t* init_disp_left,
               const sgm_roi* roi_left, const sgm_roi* roi_right);
 
-/* Same pipeline on device-resident buffers, enqueued on `stream`; no host
- * synchronisation.  disp_right may be NULL. */
+/* The plain pass on device-resident buffers, enqueued on `stream`; no host
+ * synchronisation.  The kernels read the images and write the maps in place
+ * (no staging copies).  disp_right may be NULL. */
 int sgm_match_device(sgm_handle* h, const uint8_t* d_left, const uint8_t* d_right,
                      float* d_disp_left, float* d_disp_right, void* stream);
 
@@ -110,7 +120,7 @@ int sgm_init2(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparit
 /* Replaces StereoCuda::Match2 (StereoCuda.h:69, StereoCudaImpl.cpp:548-583):
  * the sgm_match pipeline, then the depth of the final left map (as
  * sgm_depth_device) copied to depth_left.  Needs a handle set up by sgm_init2;
- * init_disp_left and the ROIs must be NULL, as for sgm_match. */
+ * init_disp_left follows the rules of sgm_match and the ROIs must be NULL. */
 int sgm_match2(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right, float* depth_left,
                const int16_t* init_disp_left, const sgm_roi* roi_left, const sgm_roi* roi_right);
 

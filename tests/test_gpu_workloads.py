@@ -61,11 +61,24 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
+def _worst_line(a, b, axis):
+    """Largest relative error over the rows (axis 0) or columns (axis 1)."""
+    d = np.moveaxis(a - b, axis, 0).reshape(a.shape[axis], -1)
+    r = np.moveaxis(b, axis, 0).reshape(b.shape[axis], -1)
+    return float((np.linalg.norm(d, axis=1) / np.maximum(np.linalg.norm(r, axis=1), 1e-30)).max())
+
+
 def test_diffsgm_full_size_against_fp64(cuda):
     """Batch 8 x 375 x 1242 x 128 with the workload's P1 ~ U[0.05, 0.2],
-    P2 ~ U[0.5, 2.0], temperature 0.1 and L1 loss; continuous random costs
-    keep the per-direction minima tie-free, so FP32 and FP64 take the same
-    argmin branches.  Images 0 and 7 against the FP64 reference."""
+    P2 ~ U[0.5, 2.0], temperature 0.1 and L1 loss, on continuous random
+    costs; images 0 and 7 against the FP64 reference.  The bar is the
+    gradients' relative error (norm of the difference over the norm of the
+    reference) <= 1e-4.  Single elements are not held to it: among the ~2e8
+    three-way minimum decisions of an image, some hundreds are within FP32
+    rounding of a tie, and FP32 and FP64 then route those elements'
+    gradients to different predecessors.  A wrong checkpoint segment or scan
+    direction would spoil whole rows or columns, so every row and column
+    must still be within 1e-2."""
     B, H, W, D, T = 8, 375, 1242, 128, 0.1
     rng = np.random.default_rng(4)
     cost = rng.random((B, H, W, D), dtype=np.float32)
@@ -88,8 +101,12 @@ def test_diffsgm_full_size_against_fp64(cuda):
         assert np.abs(d - d_ref).max() <= 1e-5 * max(1.0, np.abs(d_ref).max()), "disparity"
         for name, got, ref in (("dC", tc.grad[b], gC), ("dP1", t1.grad[b], g1), ("dP2", t2.grad[b], g2)):
             g = got.cpu().numpy().astype(np.float64)
-            assert _rel(g, ref) <= 1e-4, f"image {b} {name} rel err {_rel(g, ref):.2e}"
-            assert np.abs(g - ref).max() <= 1e-4 * np.abs(ref).max(), f"image {b} {name} max err"
+            rel, rows, cols = _rel(g, ref), _worst_line(g, ref, 0), _worst_line(g, ref, 1)
+            print(f"[dsgm full size] image {b} {name}: rel {rel:.2e}, worst row {rows:.2e}, "
+                  f"worst column {cols:.2e}, max abs {np.abs(g - ref).max():.2e} of "
+                  f"{np.abs(ref).max():.2e}", flush=True)
+            assert rel <= 1e-4, f"image {b} {name} rel err {rel:.2e}"
+            assert rows <= 1e-2 and cols <= 1e-2, f"image {b} {name} rows {rows:.2e} cols {cols:.2e}"
 
 
 def test_diffsgm_census_workload_runs_and_repeats(cuda):
