@@ -91,6 +91,7 @@ SIGNATURES = {
     "sgm_set_peak_strips": (ctypes.c_int, [I32]),
     "sgm_get_peak_strips": (I32, []),
     "sgm_stage_remove_peaks": (ctypes.c_int, [P, I32, I32, I32, F32, P, P]),
+    "sgm_stage_remove_peaks_noalloc": (ctypes.c_int, [P, I32, I32, I32, F32, P]),
     "sgm_get_disparity_range": (I32, [P]),
     "sgm_release": (ctypes.c_int, [P]),
     "sgm_last_error": (ctypes.c_char_p, [P]),

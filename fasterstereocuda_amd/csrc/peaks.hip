@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 
 #include "sgm_internal.h"
 
@@ -108,7 +109,37 @@ int peaks_threshold(int W, int strip_rows, float ratio) {
     return (int)((float)(W * strip_rows) * ratio);
 }
 
-hipError_t launch_remove_peaks(float* disp, int* scratch, int W, int H, int strips, float ratio,
+// Scratch of the allocation-free peak removal (sgm_stage_remove_peaks_noalloc):
+// parent links and segment sizes of up to kPeaksNoAllocMaxPixels pixels,
+// 2 * 4 B per pixel (16 MiB).  Each device holds its own copy for the life of
+// the process.  Host code never uses this name as an address: it passes the
+// per-device pointer from hipGetSymbolAddress.
+static __device__ __align__(256) int scratch[2 * (size_t)kPeaksNoAllocMaxPixels];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t peaks_noalloc_scratch(int** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        // once per device, with that device current: its copy's address
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<int*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_remove_peaks(float* disp, int* work, int W, int H, int strips, float ratio,
                                hipStream_t s) {
     if (W <= 0 || H <= 0 || strips <= 0) return hipErrorInvalidValue;
     const int strip_rows = H / strips;
@@ -116,8 +147,8 @@ hipError_t launch_remove_peaks(float* disp, int* scratch, int W, int H, int stri
     if (strip_rows == 0) return hipSuccess;
     const int threshold = peaks_threshold(W, strip_rows, ratio);
     const int n = W * rows;
-    int* parent = scratch;
-    int* size = scratch + (size_t)W * H;
+    int* parent = work;
+    int* size = work + (size_t)W * H;
     const unsigned blocks = (unsigned)std::min((n + 255) / 256, 65535);
     hipLaunchKernelGGL(peaks_init_kernel, dim3(blocks), dim3(256), 0, s, parent, size, n);
     hipLaunchKernelGGL(peaks_union_kernel, dim3(blocks), dim3(256), 0, s, disp, parent, W, rows,

@@ -1274,4 +1274,21 @@ int sgm_stage_remove_peaks(float* d_disp, int32_t width, int32_t height, int32_t
     return SGM_OK;
 }
 
+int sgm_stage_remove_peaks_noalloc(float* d_disp, int32_t width, int32_t height, int32_t strips,
+                                   float ratio, void* stream) {
+    if (!d_disp || width <= 0 || height <= 0 || strips < 1 || !(ratio >= 0.0f && ratio <= 1.0f))
+        return fail(nullptr, SGM_E_INVALID_ARG, "remove_peaks_noalloc: bad arguments");
+    if ((int64_t)width * height > kPeaksNoAllocMaxPixels)
+        return fail(nullptr, SGM_E_INVALID_ARG,
+                    "remove_peaks_noalloc: the map has more than 2097152 pixels (the fixed "
+                    "scratch holds 2^21); use sgm_stage_remove_peaks with a workspace");
+    int* work = nullptr;
+    HIP_TRY(nullptr, peaks_noalloc_scratch(&work));
+    // every launch starts by writing the parent and size entries it reads, so
+    // what an earlier call left in the scratch does not matter
+    HIP_TRY(nullptr, launch_remove_peaks(d_disp, work, width, height, strips, ratio,
+                                         static_cast<hipStream_t>(stream)));
+    return SGM_OK;
+}
+
 }  // extern "C"

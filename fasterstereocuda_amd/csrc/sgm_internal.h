@@ -168,6 +168,10 @@ hipError_t launch_post(const StagePlan& p, hipStream_t s);
 hipError_t launch_right_invalidate(const StagePlan& p, hipStream_t s);
 // peaks.hip: segments of fewer than (int)((float)(W r) ratio) pixels of each
 // r = H / strips row strip become invalid (in place); scratch = 2 W H ints
+// Largest map (pixels) the allocation-free peak removal takes, and the
+// current device's address of its fixed scratch (looked up once per device).
+constexpr int kPeaksNoAllocMaxPixels = 1 << 21;
+hipError_t peaks_noalloc_scratch(int** out);
 hipError_t launch_remove_peaks(float* disp, int* scratch, int W, int H, int strips, float ratio,
                                hipStream_t s);
 int peaks_threshold(int W, int strip_rows, float ratio);

@@ -348,6 +348,17 @@ int sgm_stage_post(const float* d_raw_left, const float* d_raw_right, int32_t wi
 int sgm_stage_remove_peaks(float* d_disp, int32_t width, int32_t height, int32_t strips,
                            float ratio, int32_t* d_scratch, void* stream);
 
+/* No counterpart in the reference.  sgm_stage_remove_peaks without the
+ * workspace, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * union-find scratch is a fixed array inside the library, one per device,
+ * for maps of at most 2^21 = 2097152 pixels (16 MiB, held until the process
+ * ends); a larger map returns SGM_E_INVALID_ARG.  No allocation and no host
+ * synchronisation: the work is enqueued on `stream`.  Calls on one device
+ * share that scratch, so calls that overlap in time on one device race. */
+int sgm_stage_remove_peaks_noalloc(float* d_disp, int32_t width, int32_t height, int32_t strips,
+                                   float ratio, void* stream);
+
 /* FasterStereoCuda.cpp:114-136: d_dst[(h/2)][(w/2)] = d_src[2i][2j]. */
 int sgm_stage_zoom(const uint8_t* d_src, int32_t w, int32_t h, uint8_t* d_dst, void* stream);
 /* FasterStereoCuda.cpp:138-166: base disparities of a w x h level from the
