@@ -424,7 +424,11 @@ int check_ready(sgm_handle* h) {
 // Which pass a match with (init != NULL) or without an initial disparity map
 // runs on this handle, or the error that refuses it.
 int check_match_kind(sgm_handle* h, const void* init, const sgm_roi* rl, const sgm_roi* rr) {
-    if (rl || rr) return fail(&h->err, SGM_E_UNSUPPORTED, "ROIs are not built");
+    if ((rl || rr) && init)
+        return fail(&h->err, SGM_E_UNSUPPORTED,
+                    "regions of interest with an initial disparity map are not built");
+    if ((rl && (rl->w < 0 || rl->h < 0)) || (rr && (rr->w < 0 || rr->h < 0)))
+        return fail(&h->err, SGM_E_INVALID_ARG, "a region of interest has a negative size");
     if (!init) {
         if (h->window)
             return fail(&h->err, SGM_E_UNSUPPORTED,
@@ -446,6 +450,9 @@ int check_match_kind(sgm_handle* h, const void* init, const sgm_roi* rl, const s
                     "initial disparities with min_disparity > 0 (a column offset) are not built");
     return SGM_OK;
 }
+
+// the plain pass with regions of interest (defined with the stage helpers)
+int run_roi(sgm_handle* h, const sgm_roi* rl, const sgm_roi* rr, bool want_right, hipStream_t s);
 
 }  // namespace
 
@@ -524,7 +531,11 @@ int sgm_match(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
     HIP_TRY(&h->err, hipMemcpyAsync(h->view[1].img, img_right, npix, hipMemcpyHostToDevice, s));
     if (init_disp_left)
         HIP_TRY(&h->err, hipMemcpyAsync(h->init, init_disp_left, npix * 2, hipMemcpyHostToDevice, s));
-    if (int rc = run_pipeline(h, disp_right != nullptr, s)) return rc;
+    if (roi_left || roi_right) {
+        if (int rc = run_roi(h, roi_left, roi_right, disp_right != nullptr, s)) return rc;
+    } else if (int rc = run_pipeline(h, disp_right != nullptr, s)) {
+        return rc;
+    }
     HIP_TRY(&h->err, hipMemcpyAsync(disp_left, h->view[0].out, npix * 4, hipMemcpyDeviceToHost, s));
     if (disp_right)
         HIP_TRY(&h->err, hipMemcpyAsync(disp_right, h->view[1].out, npix * 4,
@@ -566,7 +577,11 @@ int sgm_match2(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
     HIP_TRY(&h->err, hipMemcpyAsync(h->view[1].img, img_right, npix, hipMemcpyHostToDevice, s));
     if (init_disp_left)
         HIP_TRY(&h->err, hipMemcpyAsync(h->init, init_disp_left, npix * 2, hipMemcpyHostToDevice, s));
-    if (int rc = run_pipeline(h, false, s)) return rc;
+    if (roi_left || roi_right) {
+        if (int rc = run_roi(h, roi_left, roi_right, false, s)) return rc;
+    } else if (int rc = run_pipeline(h, false, s)) {
+        return rc;
+    }
     // the raw left map is free once the post stage has read it
     float* depth = h->view[0].raw;
     HIP_TRY(&h->err, launch_depth(h->view[0].out, depth, npix, h->cam.baseline * h->cam.focus,
@@ -1118,6 +1133,70 @@ struct TmpPlan {
     StagePlan plan;
     ~TmpPlan() { if (block) (void)hipFree(block); }
 };
+
+// A view's region of interest (StereoROI_T x, y, w, h; NULL = the whole
+// matched region) clipped to the image and to the columns with a cost.
+struct Rect {
+    int x0, y0, x1, y1;
+    bool any() const { return x0 < x1 && y0 < y1; }
+};
+
+Rect roi_rect(const sgm_roi* r, int W, int H, int min_d) {
+    const int lo = std::max(0, min_d);
+    if (!r) return {lo, 0, W, H};
+    return {std::max(r->x, lo), std::max(r->y, 0),
+            (int)std::min<int64_t>((int64_t)r->x + r->w, W),
+            (int)std::min<int64_t>((int64_t)r->y + r->h, H)};
+}
+
+// StereoCudaImpl::Match with regions of interest (StereoCudaImpl.cpp:418-446,
+// 463-498; cusgm_ca.cu:3984-4000 and 4180-4189): the census and each view's
+// cost cover the whole image; the view's rectangle is then packed into a
+// cost volume of its own and its four directions and WTA run over that block
+// only (each path starts on the rectangle's edge; the WTA's border rules use
+// image coordinates through the job's frame), the rest of the raw map being
+// INVALID; the LR check and the median cover the whole maps and peak removal
+// the left rectangle's rows.  Uses the stored-cost kernels.
+int run_roi(sgm_handle* h, const sgm_roi* rl, const sgm_roi* rr, bool want_right, hipStream_t s) {
+    const bool lr = h->opt.do_lr_check != 0;
+    const int W = h->W, H = h->H, D = h->D;
+    HIP_TRY(&h->err, launch_census(h->plan, s));
+    TmpPlan plans[4];
+    Rect left_rect{0, 0, 0, 0};
+    for (int v = 0; v < (lr ? 2 : 1); ++v) {
+        const Rect r = roi_rect(v == 0 ? rl : rr, W, H, h->min_d);
+        if (v == 0) left_rect = r;
+        float* raw = h->view[v].raw;
+        HIP_TRY(&h->err, launch_invalid_columns(raw, W, H, W, s));
+        if (!r.any()) continue;
+        // the view's cost over the whole image
+        const SgmJob full = make_job(h, v);
+        if (int rc = upload_plan({full}, {}, &plans[2 * v].plan, &plans[2 * v].block, &h->err)) return rc;
+        HIP_TRY(&h->err, launch_cost(plans[2 * v].plan, s));
+        // the rectangle, packed into the other view's cost buffer (free: the
+        // right view's cost is computed after the left view is done)
+        uint8_t* packed = h->view[1 - v].cost;
+        const int wc = r.x1 - r.x0, hc = r.y1 - r.y0;
+        HIP_TRY(&h->err, launch_crop_volume(h->view[v].cost, packed, W, D, r.x0, r.y0, wc, hc, s));
+        SgmJob c = full;
+        c.cost = packed;
+        c.W = wc; c.H = hc; c.roi_x = 0;
+        c.fx0 = r.x0; c.fy0 = r.y0; c.fW = W; c.fH = H;
+        if (int rc = upload_plan({c}, {}, &plans[2 * v + 1].plan, &plans[2 * v + 1].block, &h->err))
+            return rc;
+        HIP_TRY(&h->err, launch_aggregate(plans[2 * v + 1].plan, s));
+        HIP_TRY(&h->err, launch_wta(plans[2 * v + 1].plan, s));
+    }
+    HIP_TRY(&h->err, launch_post(h->plan, s));
+    if (lr && want_right) HIP_TRY(&h->err, launch_right_invalidate(h->plan, s));
+    if (h->opt.do_remove_peaks && left_rect.any())
+        HIP_TRY(&h->err, launch_remove_peaks(h->view[0].out + (size_t)left_rect.y0 * W, h->peaks, W,
+                                             left_rect.y1 - left_rect.y0, peak_strips(),
+                                             h->opt.peaks_ratio_threshold, s));
+    // the plans' device tables are freed on return
+    HIP_TRY(&h->err, hipStreamSynchronize(s));
+    return SGM_OK;
+}
 
 SgmJob stage_job(int32_t W, int32_t H, int32_t min_d, int32_t D) {
     SgmJob j{};

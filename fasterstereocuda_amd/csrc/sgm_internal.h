@@ -37,6 +37,11 @@ struct SgmJob {
     int32_t cs_type;               // 0 = 5x5, 1 = 9x7
     int32_t p1, p2;
     float uniqueness;
+    // WTA frame: the job is the block of an fW x fH image whose (0, 0) is
+    // image pixel (fx0, fy0) -- a region of interest.  fW = 0: the job is
+    // the whole image.  Border rules then use image coordinates and the map
+    // written is the image's (row pitch fW).
+    int32_t fx0, fy0, fW, fH;
 };
 
 struct SgmPost {                   // one stereo pair: LR check + median
@@ -177,6 +182,10 @@ hipError_t launch_remove_peaks(float* disp, int* scratch, int W, int H, int stri
 int peaks_threshold(int W, int strip_rows, float ratio);
 int peak_strips();   // process-wide strip count (sgm_set_peak_strips)
 // depth = (float)(f1 / ((double)disp + f2)), invalid kept (n pixels)
+// rows [y0, y0 + hc) x columns [x0, x0 + wc) of a [H][W][D] u8 volume copied
+// into a packed [hc][wc][D] volume (D a multiple of 16)
+hipError_t launch_crop_volume(const uint8_t* src, uint8_t* dst, int W, int D, int x0, int y0,
+                              int wc, int hc, hipStream_t s);
 // columns [0, ncols) of a W x H float map set to INVALID
 hipError_t launch_invalid_columns(float* disp, int W, int H, int ncols, hipStream_t s);
 hipError_t launch_depth(const float* disp, float* depth, size_t n, double f1, double f2,

@@ -506,11 +506,13 @@ __device__ __forceinline__ void wta_pixel(const SgmJob& J, int x, int y, int l,
     for (int k = 0; k < NP; ++k) s32[l * NP + k] = S[k];
     __syncthreads();
     if (l == 0 && inimg) {
-        const int W = J.W, H = J.H;
+        // rows are counted in the job; the border is the image's (the frame)
+        const int W = J.fW ? J.fW : J.W, H = J.fW ? J.fH : J.H;
+        const int xi = x + J.fx0, yi = y + J.fy0;
         constexpr bool warp_rule = (D == 64 || D == 128);
-        const int rows_cov = warp_rule ? 2 * (H / 2) : H;
+        const int rows_cov = warp_rule ? 2 * (J.H / 2) : J.H;
         float outv = kInvalidDisp;
-        if (x >= J.roi_x && y < rows_cov && !(x > W - 5 || y > H - 5 || x < 5 || y < 5)) {
+        if (x >= J.roi_x && y < rows_cov && !(xi > W - 5 || yi > H - 5 || xi < 5 || yi < 5)) {
             const uint32_t c2 = best >> 16;
             const int ds = (int)(best & 0xFFFFu);
             if (ds > 0 && (warp_rule || ds < D - 1)) {
@@ -529,7 +531,7 @@ __device__ __forceinline__ void wta_pixel(const SgmJob& J, int x, int y, int l,
                 }
             }
         }
-        gst(J.disp + (int64_t)y * W + x, outv);
+        gst(J.disp + (int64_t)yi * W + xi, outv);
     }
 }
 
@@ -674,6 +676,19 @@ __global__ __launch_bounds__(256) void depth_kernel(const float* __restrict__ di
     }
 }
 
+// A rectangle of a cost volume, packed: 16 bytes per lane.
+__global__ __launch_bounds__(256) void crop_volume_kernel(const uint8_t* __restrict__ src,
+                                                          uint8_t* __restrict__ dst, int W, int D,
+                                                          int x0, int y0, int wc, int hc) {
+    const int64_t row = (int64_t)wc * D / 16;          // 16-byte chunks per packed row
+    const int64_t n = row * hc;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t y = i / row, r = i - y * row;
+        const uint4 v = *reinterpret_cast<const uint4*>(src + ((y + y0) * W + x0) * D + r * 16);
+        *reinterpret_cast<uint4*>(dst + i * 16) = v;
+    }
+}
+
 // Columns [0, ncols) of every row set to INVALID: the part of a raw map left
 // of the matched region, which the fused on-the-fly WTA does not write.
 __global__ __launch_bounds__(256) void invalid_columns_kernel(float* __restrict__ disp, int W,
@@ -686,6 +701,16 @@ __global__ __launch_bounds__(256) void invalid_columns_kernel(float* __restrict_
 }
 
 }  // namespace
+
+hipError_t launch_crop_volume(const uint8_t* src, uint8_t* dst, int W, int D, int x0, int y0,
+                              int wc, int hc, hipStream_t s) {
+    if (wc <= 0 || hc <= 0 || D % 16 != 0) return wc <= 0 || hc <= 0 ? hipSuccess : hipErrorInvalidValue;
+    const int64_t n = (int64_t)wc * hc * D / 16;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65535);
+    hipLaunchKernelGGL(crop_volume_kernel, dim3(blocks), dim3(256), 0, s, src, dst, W, D, x0, y0,
+                       wc, hc);
+    return hipGetLastError();
+}
 
 hipError_t launch_invalid_columns(float* disp, int W, int H, int ncols, hipStream_t s) {
     ncols = std::min(ncols, W);

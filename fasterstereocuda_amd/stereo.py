@@ -111,6 +111,17 @@ def get_cost_mode() -> int:
     return int(lib().sgm_get_cost_mode())
 
 
+def _roi_ref(r):
+    """A region of interest as the C ABI takes it: None, or a pointer to an
+    sgm_roi built from an SgmRoi or an (x, y, w, h) sequence."""
+    if r is None:
+        return None
+    if not isinstance(r, SgmRoi):
+        x, y, w, h = (int(v) for v in r)
+        r = SgmRoi(x, y, w, h)
+    return ctypes.byref(r)
+
+
 class StereoCuda:
     """Single-level SGM matcher (StereoCuda.h)."""
 
@@ -150,7 +161,10 @@ class StereoCuda:
               ste_roi_left=None, ste_roi_right=None) -> bool:
         """init_disp_left (int16, the image shape) makes a disp_range 32
         handle search 32 disparities around each pixel's value, as the
-        reference's finer pyramid levels do."""
+        reference's finer pyramid levels do.  ste_roi_left / ste_roi_right
+        (SgmRoi or an (x, y, w, h) tuple) restrict the left view's and the
+        LR check's right view's matching to a rectangle; outside it the map
+        is INVALID."""
         shape = (self.height, self.width)
         host_array(img_left, shape, np.uint8, "img_left")
         host_array(img_right, shape, np.uint8, "img_right")
@@ -161,8 +175,7 @@ class StereoCuda:
             host_array(init_disp_left, shape, np.int16, "init_disp_left")
         ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
         init = ptr(init_disp_left)
-        rl = None if ste_roi_left is None else ctypes.byref(ste_roi_left)
-        rr = None if ste_roi_right is None else ctypes.byref(ste_roi_right)
+        rl, rr = _roi_ref(ste_roi_left), _roi_ref(ste_roi_right)
         return self._ok(lib().sgm_match(self._h, ptr(img_left), ptr(img_right), ptr(disp_left),
                                         ptr(disp_right), init, rl, rr))
 
@@ -188,8 +201,7 @@ class StereoCuda:
         if init_disp_left is not None:
             host_array(init_disp_left, shape, np.int16, "init_disp_left")
         init = None if init_disp_left is None else init_disp_left.ctypes.data
-        rl = None if ste_roi_left is None else ctypes.byref(ste_roi_left)
-        rr = None if ste_roi_right is None else ctypes.byref(ste_roi_right)
+        rl, rr = _roi_ref(ste_roi_left), _roi_ref(ste_roi_right)
         return self._ok(lib().sgm_match2(self._h, img_left.ctypes.data, img_right.ctypes.data,
                                          depth_left.ctypes.data, init, rl, rr))
 

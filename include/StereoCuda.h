@@ -38,8 +38,11 @@ public:
      * init_disp_left (sint16, INVALID = get_invad_short()) is taken by an
      * object initialised with disp_range 32, do_lr_check false and
      * using_constant_p2: it matches in a 32-disparity window starting at each
-     * pixel's initial disparity, as the coarse-to-fine levels do.  The ROIs
-     * must be nullptr: ROI-restricted matching is not built (returns false). */
+     * pixel's initial disparity, as the coarse-to-fine levels do.
+     * ste_roi_left / ste_roi_right restrict each view's aggregation and WTA to
+     * that rectangle (clipped to the image; paths start on its edges); pixels
+     * outside it are INVALID.  A region together with init_disp_left is
+     * refused (returns false). */
     bool Match(uint8* img_left, uint8* img_right, float32* disp_left,
                float32* disp_right = nullptr, sint16* init_disp_left = nullptr,
                StereoROI_T* ste_roi_left = nullptr, StereoROI_T* ste_roi_right = nullptr) const;

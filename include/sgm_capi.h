@@ -84,8 +84,12 @@ int sgm_init(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparity
  *     aggregation, window WTA, median; StereoCudaImpl.cpp:363-446,
  *     cusgm_ca.cu:4052-4077).  That needs do_lr_check = 0, using_constant_p2
  *     and min_disparity <= 0; disp_right, if given, is all INVALID.
- * The ROIs must be NULL (SGM_E_UNSUPPORTED): ROI-restricted matching is not
- * built. */
+ * roi_left / roi_right (either may be NULL) restrict that view's aggregation
+ * and WTA to the rectangle clipped to the image: the paths start on its
+ * edges, the WTA's border rules stay those of the image, pixels outside are
+ * INVALID, and the LR check, median and peak removal run on the full maps
+ * (peaks over the left rectangle's rows).  A negative width or height is
+ * SGM_E_INVALID_ARG; a region with init_disp_left is SGM_E_UNSUPPORTED. */
 int sgm_match(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right,
               float* disp_left, float* disp_right, const int16_t* init_disp_left,
               const sgm_roi* roi_left, const sgm_roi* roi_right);
@@ -120,7 +124,7 @@ int sgm_init2(sgm_handle* h, int32_t width, int32_t height, int32_t min_disparit
 /* Replaces StereoCuda::Match2 (StereoCuda.h:69, StereoCudaImpl.cpp:548-583):
  * the sgm_match pipeline, then the depth of the final left map (as
  * sgm_depth_device) copied to depth_left.  Needs a handle set up by sgm_init2;
- * init_disp_left follows the rules of sgm_match and the ROIs must be NULL. */
+ * init_disp_left and the ROIs follow the rules of sgm_match. */
 int sgm_match2(sgm_handle* h, const uint8_t* img_left, const uint8_t* img_right, float* depth_left,
                const int16_t* init_disp_left, const sgm_roi* roi_left, const sgm_roi* roi_right);
 
@@ -239,15 +243,18 @@ int sgm_batch_use_graph(sgm_batch* b, int32_t on);
  * image with the textbook boundary (no d-1 term at d = 0, no d+1 term at
  * d = D-1).  D must be 64, 128 or 256.  All buffers are device memory owned
  * by the caller; sizes in floats come from the two query functions.  The
- * forward stores S (needed by the backward) and per-direction checkpoints
- * every 32 scanline steps instead of the full per-direction volumes. */
+ * forward stores S (needed by the backward), per-direction checkpoints every
+ * 32 scanline steps and two floats per pixel in d_ckpt; no per-direction
+ * volume is kept.  The transient workspace (d_work, shared by forward and
+ * backward) is one direction volume plus two per-pixel planes. */
 size_t sgm_dsgm_checkpoint_floats(int32_t B, int32_t H, int32_t W, int32_t D);
 size_t sgm_dsgm_workspace_floats(int32_t B, int32_t H, int32_t W, int32_t D);
 int sgm_dsgm_forward(const float* d_cost, const float* d_p1, const float* d_p2, int32_t B,
                      int32_t H, int32_t W, int32_t D, float temperature, float* d_S,
                      float* d_disp, float* d_ckpt, float* d_work, void* stream);
-/* Gradients of a scalar loss given dLoss/d disp.  d_grad_cost doubles as
- * scratch for dLoss/dS until the last kernel writes the result. */
+/* Gradients of a scalar loss given dLoss/d disp.  dLoss/dS is formed from S
+ * where it is used; d_grad_cost, d_grad_p1 and d_grad_p2 are built in place
+ * over the directions. */
 int sgm_dsgm_backward(const float* d_cost, const float* d_p1, const float* d_p2,
                       const float* d_S, const float* d_disp, const float* d_ckpt,
                       const float* d_grad_disp, int32_t B, int32_t H, int32_t W, int32_t D,

@@ -63,6 +63,8 @@ def lib():
         L.orc_median3x3.argtypes = [P, P, I, I]
         L.orc_match.argtypes = [P, P, I, I, I, I, ctypes.POINTER(Options), P, P]
         L.orc_match.restype = I
+        L.orc_match_roi.argtypes = [P, P, I, I, I, I, ctypes.POINTER(Options), P, P, P, P]
+        L.orc_match_roi.restype = I
         _lib = L
     return _lib
 
@@ -142,6 +144,28 @@ def match(left: np.ndarray, right: np.ndarray, min_d: int, D: int, cs_type: int 
     rc = lib().orc_match(_p(left), _p(right), W, H, min_d, D, ctypes.byref(opt), _p(dl), _p(dr))
     if rc != 0:
         raise RuntimeError(f"orc_match failed with {rc}")
+    return dl, dr
+
+
+def match_roi(left: np.ndarray, right: np.ndarray, min_d: int, D: int, roi_left=None,
+              roi_right=None, cs_type: int = 1, p1: int = 10, p2: int = 150,
+              uniqueness: float = 0.95, do_lr_check: bool = True, lr_threshold: float = 1.0,
+              do_median: bool = True, peaks_ratio: float | None = None, peak_strips: int = 1):
+    """The plain pipeline with a region of interest (x, y, w, h) per view
+    (None = the whole matched region).  Returns (disp_left, disp_right)."""
+    left = np.ascontiguousarray(left, dtype=np.uint8)
+    right = np.ascontiguousarray(right, dtype=np.uint8)
+    H, W = left.shape
+    opt = Options(cs_type, p1, p2, uniqueness, int(do_lr_check), lr_threshold, int(do_median),
+                  int(peaks_ratio is not None), peaks_ratio or 0.0, peak_strips)
+    rl = None if roi_left is None else np.ascontiguousarray(roi_left, np.int32)
+    rr = None if roi_right is None else np.ascontiguousarray(roi_right, np.int32)
+    dl = np.empty((H, W), np.float32)
+    dr = np.empty((H, W), np.float32)
+    rc = lib().orc_match_roi(_p(left), _p(right), W, H, min_d, D, ctypes.byref(opt), _p(rl), _p(rr),
+                             _p(dl), _p(dr))
+    if rc != 0:
+        raise RuntimeError(f"orc_match_roi failed with {rc}")
     return dl, dr
 
 

@@ -216,23 +216,30 @@ void orc_aggregate(const uint8_t* cost, int W, int H, int D, int roi_x,
  *          den = max(1, c1 + c3 - 2 c2)
  * A5: the D=64/128 kernels cover rows [0, 2*(H/2)) and the generic kernel
  * all rows; columns below roi_x are never written.  Uncovered -> INVALID. */
-void orc_wta(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2,
-             const uint8_t* p3, int W, int H, int D, int min_d, int roi_x,
-             float uniqueness, float* disp) {
+/* WTA of a block of Wc x Hc pixels whose (0, 0) is image pixel (x0, y0) of
+ * a W x H image (the whole image, or the rectangle of a region of interest):
+ * the border and disparity rules use image coordinates, the covered rows
+ * (2 (Hc/2) for D = 64 / 128, the reference's 2-row WTA grid over the
+ * region's height) are counted from the block's first row, and block columns
+ * below roi_x are INVALID.  disp is the image map (row pitch W). */
+static void wta_block(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2,
+                      const uint8_t* p3, int Wc, int Hc, int D, int min_d, int roi_x, int x0,
+                      int y0, int W, int H, float uniqueness, float* disp) {
     const bool warp_rule = (D == 64 || D == 128);
-    const int rows_covered = warp_rule ? 2 * (H / 2) : H;
+    const int rows_covered = warp_rule ? 2 * (Hc / 2) : Hc;
     const float unique_threshold = 1.0f - uniqueness;
 #pragma omp parallel
     {
         std::vector<uint32_t> S(D);
 #pragma omp for schedule(static)
-        for (int y = 0; y < H; ++y) {
-            for (int x = 0; x < W; ++x) {
-                float& out = disp[idx2(x, y, W)];
+        for (int y = 0; y < Hc; ++y) {
+            for (int x = 0; x < Wc; ++x) {
+                const int xi = x + x0, yi = y + y0;
+                float& out = disp[idx2(xi, yi, W)];
                 out = ORC_INVALID_DISP;
                 if (x < roi_x || y >= rows_covered) continue;
-                if (x > W - 5 || y > H - 5 || x < 5 || y < 5) continue;
-                const size_t base = idx3(x, y, 0, W, D);
+                if (xi > W - 5 || yi > H - 5 || xi < 5 || yi < 5) continue;
+                const size_t base = idx3(x, y, 0, Wc, D);
                 uint32_t c2 = 0xFFFFFFFFu;
                 int dstar = 0;
                 for (int d = 0; d < D; ++d) {
@@ -254,6 +261,12 @@ void orc_wta(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2,
             }
         }
     }
+}
+
+void orc_wta(const uint8_t* p0, const uint8_t* p1, const uint8_t* p2,
+             const uint8_t* p3, int W, int H, int D, int min_d, int roi_x,
+             float uniqueness, float* disp) {
+    wta_block(p0, p1, p2, p3, W, H, D, min_d, roi_x, 0, 0, W, H, uniqueness, disp);
 }
 
 /* Left-right consistency check.  cusgm_ca.cu:3873-3898.
@@ -707,6 +720,89 @@ int orc_match(const uint8_t* left, const uint8_t* right, int W, int H,
     }
     // after the filters, on the left map only (StereoCudaImpl.cpp:541-542)
     if (opt->do_remove_peaks) orc_remove_peaks(disp_l, W, H, opt->peak_strips, opt->peaks_ratio);
+    if (disp_r) std::memcpy(disp_r, dr.data(), npix * sizeof(float));
+    return 0;
+}
+
+/* The region of interest of one view (StereoROI_T x, y, w, h; NULL = the
+ * whole matched region), clipped to the image and to the columns that have a
+ * cost (x >= max(0, min_d)).  Returns false when nothing is left. */
+static bool roi_rect(const int* roi, int W, int H, int min_d, int* x0, int* y0, int* x1,
+                     int* y1) {
+    const int lo = std::max(0, min_d);
+    if (!roi) {
+        *x0 = lo; *y0 = 0; *x1 = W; *y1 = H;
+    } else {
+        *x0 = std::max(roi[0], lo);
+        *y0 = std::max(roi[1], 0);
+        *x1 = (int)std::min<long long>((long long)roi[0] + roi[2], W);
+        *y1 = (int)std::min<long long>((long long)roi[1] + roi[3], H);
+    }
+    return *x0 < *x1 && *y0 < *y1;
+}
+
+/* One view restricted to a rectangle: its four directions run over the
+ * rectangle only (each path's first pixel on the rectangle's edge), the WTA
+ * fills the rectangle (image-coordinate border rules, rows counted from its
+ * top) and every other pixel of disp is INVALID. */
+static void match_rect(const uint8_t* cost, int W, int H, int D, int min_d, int x0, int y0,
+                       int x1, int y1, const orc_options* opt, float* disp) {
+    for (size_t i = 0; i < (size_t)W * H; ++i) disp[i] = ORC_INVALID_DISP;
+    const int wc = x1 - x0, hc = y1 - y0;
+    const size_t ncell = (size_t)wc * hc * D;
+    std::vector<uint8_t> sub(ncell);
+    for (int y = 0; y < hc; ++y)
+        std::memcpy(&sub[idx3(0, y, 0, wc, D)], &cost[idx3(x0, y + y0, 0, W, D)], (size_t)wc * D);
+    std::vector<uint8_t> path[4];
+    for (int r = 0; r < 4; ++r) {
+        path[r].assign(ncell, 0);
+        orc_aggregate(sub.data(), wc, hc, D, 0, r, opt->p1, opt->p2, path[r].data());
+    }
+    wta_block(path[0].data(), path[1].data(), path[2].data(), path[3].data(), wc, hc, D, min_d,
+              0, x0, y0, W, H, opt->uniqueness, disp);
+}
+
+/* StereoCudaImpl::Match with regions of interest (StereoCudaImpl.cpp:418-446,
+ * 463-498; cusgm_ca.cu:3984-4000 left, 4180-4189 right).  The costs are
+ * those of the whole image; each view's directions and WTA are restricted to
+ * its rectangle (roi_l for the left view, roi_r for the right view of the LR
+ * check; NULL = the whole matched region); the LR check and the median run
+ * over the whole maps; peak removal splits the left rectangle's rows into
+ * strips.  With both NULL this is orc_match.  A21: outside a rectangle the
+ * map is INVALID (the reference leaves whatever its buffer held). */
+int orc_match_roi(const uint8_t* left, const uint8_t* right, int W, int H, int min_d, int D,
+                  const orc_options* opt, const int* roi_l, const int* roi_r, float* disp_l,
+                  float* disp_r) {
+    if (!left || !right || !disp_l || !opt || W <= 0 || H <= 0 || D <= 0) return -1;
+    const size_t npix = (size_t)W * H;
+    const size_t ncell = npix * (size_t)D;
+    std::vector<uint64_t> csl(npix), csr(npix);
+    std::vector<uint8_t> cost(ncell), l2r;
+    std::vector<float> dl(npix, ORC_INVALID_DISP), dr(npix, ORC_INVALID_DISP);
+    orc_census(left, W, H, opt->cs_type, 0, csl.data());
+    orc_census(right, W, H, opt->cs_type, 1, csr.data());
+    orc_cost(csl.data(), csr.data(), W, H, min_d, D, 1, opt->cs_type, nullptr, cost.data());
+    int lx0, ly0, lx1, ly1;
+    const bool left_any = roi_rect(roi_l, W, H, min_d, &lx0, &ly0, &lx1, &ly1);
+    if (left_any) match_rect(cost.data(), W, H, D, min_d, lx0, ly0, lx1, ly1, opt, dl.data());
+    if (opt->do_lr_check) {
+        if (opt->cs_type == 0) l2r = cost;
+        orc_cost(csr.data(), csl.data(), W, H, min_d, D, 0, opt->cs_type,
+                 opt->cs_type == 0 ? l2r.data() : nullptr, cost.data());
+        std::vector<float> dr_in(npix, ORC_INVALID_DISP);
+        int rx0, ry0, rx1, ry1;
+        if (roi_rect(roi_r, W, H, min_d, &rx0, &ry0, &rx1, &ry1))
+            match_rect(cost.data(), W, H, D, min_d, rx0, ry0, rx1, ry1, opt, dr_in.data());
+        dr = dr_in;
+        orc_lrcheck(dl.data(), dr_in.data(), dr.data(), W, H, opt->lr_threshold);
+    }
+    if (opt->do_median) {
+        orc_median3x3(dl.data(), disp_l, W, H);
+    } else {
+        std::memcpy(disp_l, dl.data(), npix * sizeof(float));
+    }
+    if (opt->do_remove_peaks && left_any)
+        orc_remove_peaks(disp_l + (size_t)ly0 * W, W, ly1 - ly0, opt->peak_strips, opt->peaks_ratio);
     if (disp_r) std::memcpy(disp_r, dr.data(), npix * sizeof(float));
     return 0;
 }

@@ -119,6 +119,12 @@ int orc_match(const uint8_t* left, const uint8_t* right, int W, int H,
               int min_d, int D, const orc_options* opt, float* disp_l,
               float* disp_r);
 
+/* orc_match with a region of interest per view (x, y, w, h; NULL = the whole
+ * matched region); see sgm_oracle.cpp. */
+int orc_match_roi(const uint8_t* left, const uint8_t* right, int W, int H, int min_d, int D,
+                  const orc_options* opt, const int* roi_l, const int* roi_r, float* disp_l,
+                  float* disp_r);
+
 #ifdef __cplusplus
 }
 #endif

@@ -804,3 +804,108 @@ def test_remove_peaks_against_union_find(strips, ratio, seed):
     exp[kill] = oracle.INVALID
     np.testing.assert_array_equal(out, exp)
     assert n == int(kill.sum()) and n > 0
+
+
+# ---------------------------------------------------------------- regions of interest
+def wta_numpy_frame(paths, min_d, x0, y0, W, H, uniq):
+    """wta_numpy over a block whose (0, 0) is image pixel (x0, y0): border
+    rules in image coordinates, covered rows counted in the block; returns
+    the W x H image map, INVALID outside the block."""
+    h, w, D = paths[0].shape
+    S = sum(p.astype(np.int64) for p in paths)
+    out = np.full((H, W), INV, np.float32)
+    warp = D in (64, 128)
+    rows = 2 * (h // 2) if warp else h
+    ut = np.float32(1.0) - np.float32(uniq)
+    for y in range(h):
+        for x in range(w):
+            xi, yi = x + x0, y + y0
+            if y >= rows or xi > W - 5 or yi > H - 5 or xi < 5 or yi < 5:
+                continue
+            s = S[y, x]
+            ds = int(np.argmin(s))
+            if ds <= 0 or (not warp and ds >= D - 1):
+                continue
+            c1, c2 = np.float32(s[ds - 1]), np.float32(s[ds])
+            c3 = np.float32(D - 1) if ds == D - 1 else np.float32(s[ds + 1])
+            if min(c1 - c2, c3 - c2) <= c2 * ut:
+                continue
+            den = max(np.float32(1.0), c1 + c3 - np.float32(2.0) * c2)
+            out[yi, xi] = np.float32(ds + min_d) + (c1 - c3 + den) / (np.float32(2.0) * den)
+    return out
+
+
+def _roi_view(left, right, min_d, D, roi, is_right, cs=1, p1=10, p2=150, uniq=0.95):
+    """One view's raw map with a region of interest, composed from the pinned
+    stages: whole-image census and cost, the rectangle cut out, the penalty-
+    matrix aggregation over it and the framed numpy WTA."""
+    H, W = left.shape
+    cl, cr = oracle.census(left, cs, False), oracle.census(right, cs, True)
+    l2r = oracle.cost(cl, cr, min_d, D, True, cs)
+    cost = l2r if not is_right else oracle.cost(cr, cl, min_d, D, False, cs, l2r if cs == 0 else None)
+    lo = max(0, min_d)
+    if roi is None:
+        x0, y0, x1, y1 = lo, 0, W, H
+    else:
+        x0, y0 = max(roi[0], lo), max(roi[1], 0)
+        x1, y1 = min(roi[0] + roi[2], W), min(roi[1] + roi[3], H)
+    if x0 >= x1 or y0 >= y1:
+        return np.full((H, W), INV, np.float32)
+    sub = np.ascontiguousarray(cost[y0:y1, x0:x1])
+    paths = [aggregate_penalty_matrix(sub, 0, r, p1, p2) for r in range(4)]
+    return wta_numpy_frame(paths, min_d, x0, y0, W, H, uniq)
+
+
+@pytest.mark.parametrize("min_d,cs", [(0, 1), (7, 1), (-5, 0), (3, 0)])
+def test_match_roi_whole_region_is_match(min_d, cs):
+    """No region, the default region and a region past every edge all give
+    orc_match exactly."""
+    p = scenes.cone_like(3, H=48, W=80, D=64)
+    exp = oracle.match(p.left, p.right, min_d, 64, cs_type=cs)
+    lo = max(0, min_d)
+    for rl, rr in ((None, None), ((lo, 0, 80 - lo, 48), None), ((-9, -4, 300, 200), (-2, -2, 99, 99))):
+        got = oracle.match_roi(p.left, p.right, min_d, 64, rl, rr, cs_type=cs)
+        for g, e in zip(got, exp):
+            np.testing.assert_array_equal(g.view(np.uint32), e.view(np.uint32))
+
+
+@pytest.mark.parametrize("roi,min_d,cs", [((7, 4, 20, 15), 0, 1), ((0, 9, 33, 12), 6, 1),
+                                          ((12, -3, 25, 40), -4, 0), ((30, 5, 0, 10), 0, 1)])
+def test_match_roi_is_the_rectangle_pipeline(roi, min_d, cs):
+    """Without LR and median the left map is the framed WTA of the four
+    directions aggregated over the rectangle of the whole-image cost."""
+    p = scenes.cone_like(4, H=30, W=40, D=64)
+    got, _ = oracle.match_roi(p.left, p.right, min_d, 64, roi, None, cs_type=cs, do_lr_check=False,
+                              do_median=False)
+    exp = _roi_view(p.left, p.right, min_d, 64, roi, False, cs=cs)
+    np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+    if roi[2] > 0:
+        assert (got != INV).any()
+        x0, y0 = max(roi[0], max(0, min_d)), max(roi[1], 0)
+        outside = np.ones(got.shape, bool)
+        outside[y0:roi[1] + roi[3], x0:roi[0] + roi[2]] = False
+        assert (got[outside] == INV).all()
+    else:
+        assert (got == INV).all()
+
+
+def test_match_roi_lr_median_and_peaks():
+    """Both views restricted to their own rectangles, then the LR check and
+    the median over the whole maps (each pinned on its own), and peak removal
+    over the left rectangle's rows in strips."""
+    p = scenes.cone_like(5, H=30, W=40, D=64)
+    rl, rr = (6, 3, 28, 22), (2, 5, 30, 20)
+    dl = _roi_view(p.left, p.right, 0, 64, rl, False)
+    dr_in = _roi_view(p.left, p.right, 0, 64, rr, True)
+    exp_l, exp_r = oracle.lrcheck(dl, dr_in, 1.0)
+    exp_l = oracle.median3x3(exp_l)
+    got_l, got_r = oracle.match_roi(p.left, p.right, 0, 64, rl, rr)
+    np.testing.assert_array_equal(got_l.view(np.uint32), exp_l.view(np.uint32))
+    np.testing.assert_array_equal(got_r.view(np.uint32), exp_r.view(np.uint32))
+    # peaks: strips over rows [3, 25) only
+    ratio, strips = 0.02, 2
+    got_p, _ = oracle.match_roi(p.left, p.right, 0, 64, rl, rr, peaks_ratio=ratio, peak_strips=strips)
+    exp_p = got_l.copy()
+    exp_p[3:25], n = oracle.remove_peaks(got_l[3:25], strips, ratio)
+    assert n > 0
+    np.testing.assert_array_equal(got_p.view(np.uint32), exp_p.view(np.uint32))
