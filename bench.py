@@ -57,6 +57,8 @@ def parse():
                     help="timed CPU baseline runs (median reported), after one warm-up")
     ap.add_argument("--cpu-pairs", type=int, default=16,
                     help="pairs of the workload per CPU baseline run")
+    ap.add_argument("--schedule", default="grid", choices=["grid", "longest-first", "autotune"],
+                    help="scanline schedule of the batched aggregation launches")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the pyramid, uneven-batch and differentiable workloads in the "
                          "default line")
@@ -105,6 +107,12 @@ def main():
             "disp_range": D,
         })
     batch = SgmBatch(pairs, opt)
+    from fasterstereocuda_amd.batch import SCHED_GRID, SCHED_LONGEST_FIRST
+    if args.schedule == "autotune":
+        batch.autotune(reps=3, force=True)
+    else:
+        batch.set_schedule({"grid": SCHED_GRID, "longest-first": SCHED_LONGEST_FIRST}[args.schedule])
+    schedule = batch.schedule
     stream = torch.cuda.current_stream()
 
     for _ in range(args.warmup):
@@ -204,6 +212,8 @@ def main():
             "workload": f"driving-like, {NP} pairs {W}x{H}, D={D}, 9x7 census, P1=10 P2=150, "
                         "uniqueness 0.95, LR check tol 1, 3x3 median",
             "pairs": NP, "width": W, "height": H, "disp_range": D, "path": "batched",
+            "schedule": {"policy": ["grid", "job-major", "longest-first"][schedule[0]],
+                         "waves": schedule[1]},
         },
         "roofline": {
             "bound": "hbm",
@@ -588,7 +598,7 @@ def diffsgm(args, quick=False):
         return {"workload": "differentiable FP32 SGM, batch 8, 1242x375, D=128, per-pixel P1/P2, "
                             "soft-argmin T=0.1, L1 loss",
                 "ms_per_pair": ms / B, "ms_per_step": ms, "unit": "ms/pair", "steps": steps,
-                "loss": float(loss), "roofline": roof}
+                "loss": float(loss.detach()), "roofline": roof}
     _emit(_common(args, "diff-SGM fwd+bwd ms/pair", ms / B, "ms/pair", ms, False,
                   {"workload": "differentiable FP32 SGM, batch 8, 1242x375, D=128, per-pixel P1/P2, "
                                "soft-argmin T=0.1, L1 loss", "batch": B, "width": W, "height": H,
@@ -613,6 +623,13 @@ def run_other(args):
     import torch
     torch.cuda.set_device(0)
     {"building": building, "uneven": uneven, "diffsgm": diffsgm}[args.workload](args)
+    if args.copy_probe:
+        # counter calibration: a float32 negate of 256 MiB (reads and writes 256 MiB)
+        a = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

@@ -27,15 +27,19 @@ int dsgm_fail(const char* msg) { return fsgm::set_global_error(SGM_E_INVALID_ARG
 
 extern "C" {
 
+// saved by the forward for the backward, in floats: the per-pixel minimum of
+// S and softmax normaliser (2 floats per pixel), the winner codes (1 byte
+// per cell over the four directions) and the argmin bytes (4 per pixel)
 size_t sgm_dsgm_checkpoint_floats(int32_t B, int32_t H, int32_t W, int32_t D) {
     if (!dsgm_shape_ok(B, H, W, D)) return 0;
-    return (size_t)B * (size_t)dsgm_ckpt_floats_per_image(H, W, D);
+    return (size_t)((dsgm_saved_bytes(B, H, W, D) + 3) / 4);
 }
 
+// transient: one direction volume and two per-pixel planes
 size_t sgm_dsgm_workspace_floats(int32_t B, int32_t H, int32_t W, int32_t D) {
     if (!dsgm_shape_ok(B, H, W, D)) return 0;
     const size_t npix = (size_t)B * H * W;
-    return 4 * npix * D + 8 * npix;
+    return npix * D + 2 * npix;
 }
 
 int sgm_dsgm_forward(const float* d_cost, const float* d_p1, const float* d_p2, int32_t B,
@@ -47,8 +51,11 @@ int sgm_dsgm_forward(const float* d_cost, const float* d_p1, const float* d_p2, 
         return dsgm_fail("dsgm: NULL pointer");
     DsgmArgs A{};
     A.cost = d_cost; A.p1 = d_p1; A.p2 = d_p2;
-    A.paths = d_work;
-    A.ckpt = d_ckpt; A.S = d_S; A.disp = d_disp;
+    A.vol = d_work;
+    A.S = d_S; A.disp = d_disp;
+    A.stats = d_ckpt;
+    A.wins = reinterpret_cast<uint8_t*>(d_ckpt) + dsgm_saved_wins_offset(B, H, W);
+    A.kstar = reinterpret_cast<uint8_t*>(d_ckpt) + dsgm_saved_kstar_offset(B, H, W, D);
     A.B = B; A.H = H; A.W = W; A.D = D;
     A.inv_temperature = 1.0f / temperature;
     const hipError_t e = dsgm_launch_forward(A, static_cast<hipStream_t>(stream));
@@ -69,14 +76,16 @@ int sgm_dsgm_backward(const float* d_cost, const float* d_p1, const float* d_p2,
     const size_t npix = (size_t)B * H * W;
     DsgmArgs A{};
     A.cost = d_cost; A.p1 = d_p1; A.p2 = d_p2;
-    A.ckpt = const_cast<float*>(d_ckpt);
+    A.stats = const_cast<float*>(d_ckpt);
+    A.wins = reinterpret_cast<uint8_t*>(const_cast<float*>(d_ckpt)) + dsgm_saved_wins_offset(B, H, W);
+    A.kstar = reinterpret_cast<uint8_t*>(const_cast<float*>(d_ckpt)) +
+              dsgm_saved_kstar_offset(B, H, W, D);
     A.S = const_cast<float*>(d_S);
     A.disp = const_cast<float*>(d_disp);
     A.grad_disp = d_grad_disp;
-    A.grad_s = d_grad_cost;
-    A.grad_paths = d_work;
-    A.grad_p1_paths = d_work + 4 * npix * D;
-    A.grad_p2_paths = d_work + 4 * npix * D + 4 * npix;
+    A.vol = d_work;
+    A.vol_p1 = d_work + npix * D;
+    A.vol_p2 = d_work + npix * D + npix;
     A.grad_cost = d_grad_cost; A.grad_p1 = d_grad_p1; A.grad_p2 = d_grad_p2;
     A.B = B; A.H = H; A.W = W; A.D = D;
     A.inv_temperature = 1.0f / temperature;

@@ -5,44 +5,39 @@
 
 namespace fsgm {
 
-constexpr int kDsgmSeg = 32;   // steps between checkpoints
-
 struct DsgmArgs {
-    const float* cost;         // [B][H][W][D]
+    const float* cost;         // [B][H][W][D] (the forward's input)
     const float* p1;           // [B][H][W]
     const float* p2;           // [B][H][W]
-    float* paths;              // [B][4][H][W][D] forward scratch (one volume per direction)
-    float* ckpt;               // checkpoints, dsgm_ckpt_floats_per_image() per image
-    float* S;                  // [B][H][W][D]
+    float* S;                  // [B][H][W][D]: sum of the four directions (built in place)
     float* disp;               // [B][H][W]
+    float* stats;              // [B][H][W][2]: minimum of S and softmax normaliser per pixel
+    uint8_t* wins;             // [4][B][H][W][16 lanes] winner codes, D / 64 bytes per lane
+    uint8_t* kstar;            // [4][B][H][W]: smallest argmin of the previous step's L
+    float* vol;                // [B][H][W][D] workspace: top->bottom L (forward), its dC (backward)
+    float* vol_p1;             // [B][H][W] workspace: top->bottom dP1 (backward)
+    float* vol_p2;             // [B][H][W] workspace: top->bottom dP2 (backward)
     const float* grad_disp;    // [B][H][W]
-    float* grad_s;             // [B][H][W][D] backward scratch
-    float* grad_paths;         // [B][4][H][W][D] backward scratch
-    float* grad_p1_paths;      // [B][4][H][W]
-    float* grad_p2_paths;      // [B][4][H][W]
-    float* grad_cost;          // [B][H][W][D]
+    float* grad_cost;          // [B][H][W][D] (built in place)
     float* grad_p1;            // [B][H][W]
     float* grad_p2;            // [B][H][W]
     int32_t B, H, W, D;
     float inv_temperature;
 };
 
-// Checkpoint rows (D floats each) before direction `path` within one image:
-// directions 0 and 2 are vertical (W lines of ceil(H/32) checkpoints),
-// 1 and 3 horizontal (H lines of ceil(W/32)).
-__host__ __device__ inline int64_t dsgm_ckpt_path_offset(int H, int W, int /*D*/, int path) {
-    const int64_t v = (int64_t)W * ((H + kDsgmSeg - 1) / kDsgmSeg);
-    const int64_t h = (int64_t)H * ((W + kDsgmSeg - 1) / kDsgmSeg);
-    switch (path) {
-        case 0: return 0;
-        case 1: return v;
-        case 2: return v + h;
-        default: return 2 * v + h;
-    }
+// What the forward saves for the backward, in one buffer: the per-pixel
+// statistics (2 floats), then the winner codes of every cell of every
+// direction (2 bits each: D / 4 bytes per pixel and direction), then each
+// pixel's argmin byte per direction.
+__host__ __device__ inline int64_t dsgm_saved_bytes(int B, int H, int W, int D) {
+    const int64_t npix = (int64_t)B * H * W;
+    return npix * 8 + npix * D + npix * 4;
 }
-__host__ __device__ inline int64_t dsgm_ckpt_floats_per_image(int H, int W, int D) {
-    return dsgm_ckpt_path_offset(H, W, D, 3) * D +
-           (int64_t)H * ((W + kDsgmSeg - 1) / kDsgmSeg) * D;
+__host__ __device__ inline int64_t dsgm_saved_wins_offset(int B, int H, int W) {
+    return (int64_t)B * H * W * 8;
+}
+__host__ __device__ inline int64_t dsgm_saved_kstar_offset(int B, int H, int W, int D) {
+    return (int64_t)B * H * W * (8 + D);
 }
 
 hipError_t dsgm_launch_forward(const DsgmArgs& A, hipStream_t s);

@@ -8,23 +8,30 @@
 //     L(p0, d) = C(p0, d)
 //     L(p, d)  = C(p, d) + min(L'(d), min(L'(d-1), L'(d+1)) + P1(p), m' + P2(p)) - m'
 //     L'(-1) = L'(D) = +inf,  m' = min_d L'(d)
-//   S = ((L_0 + L_1) + L_2) + L_3
+//   S = ((L_1 + L_3) + L_0) + L_2
 //   disp(p) = sum_d d * softmax_d(-S(p, d) / T)
 // Subgradients: the first min candidate in the order [L'(d), L'(d-1)+P1,
 // L'(d+1)+P1, m'+P2] wins; m' = L'(k*) with k* the smallest argmin.
 //
-// Memory: the forward stores the volumes of directions 1, 3 and 0; direction
-// 2 runs last and forms S from them and its own L at every pixel.  For the
-// backward the forward stores L every 32 steps of every scanline
-// (checkpoints).  The backward walks each scanline's 32-step segments from
-// the last to the first: it recomputes the segment from its checkpoint,
-// keeping the winner codes (2 bits per disparity) and k* of every step in
-// registers, then runs the reverse recurrence
+// Memory: S is built in place and no per-direction volume is kept.  The
+// forward's first launch runs, per group of 4 rows, left->right (S = L_1)
+// and then right->left (S += L_3) in one wave, so the second pass adds onto
+// the first with no other writer; beside them the top->bottom groups write
+// L_0 into the one workspace volume.  The second launch runs bottom->top,
+// forms S = (S + L_0) + L_2 and the soft-argmin, and keeps the minimum of S
+// and the softmax normaliser per pixel.  Every direction also saves, per
+// step, the winner code of each disparity (2 bits) and k* of the previous
+// step's L (one byte per pixel): 1 byte per cell over the four directions,
+// a quarter of a volume, which is all the reverse recurrence needs of the
+// forward.  The backward walks each scanline from its last step to its
+// first, once:
 //   G(s-1, d) = gS(s-1, d) + G(s, d)[w(s,d)=0] + G(s, d+1)[w(s,d+1)=1]
 //             + G(s, d-1)[w(s,d-1)=2] + [d = k*(s-1)] (sum_e G(s,e)[w(s,e)=3] - sum_e G(s,e))
-// and emits dC_r(s, d) = G(s, d), dP1_r(s) = sum_d G(s,d)[w in {1,2}],
-// dP2_r(s) = sum_d G(s,d)[w = 3].  Directions 1, 3, 0 store theirs;
-// direction 2 runs last and adds its own to them at every pixel.
+// with gS = dLoss/dS formed from S, the kept minimum and normaliser, and
+// emits dC_r(s, d) = G(s, d), dP1_r(s) = sum_d G(s,d)[w in {1,2}],
+// dP2_r(s) = sum_d G(s,d)[w = 3].  Its launches mirror the forward's: dC,
+// dP1 and dP2 are built in place by the horizontal pairs, the top->bottom
+// terms go to the workspace, and bottom->top adds both in the same order as S.
 //
 // Layout per wave as in the integer aggregation: four 16-lane DPP rows, one
 // scanline per row, D/16 disparities per lane.
@@ -32,13 +39,12 @@
 #include <cmath>
 
 #include "dsgm_internal.h"
+#include "sgm_device.h"
 
 #pragma clang fp contract(off)
 
 namespace fsgm {
 namespace {
-
-constexpr int kSeg = 32;
 
 template <int CTRL>
 __device__ __forceinline__ float dppf(float old, float v) {
@@ -92,6 +98,13 @@ __device__ __forceinline__ void storef(float* p, const float (&v)[N]) {
 }
 __device__ __forceinline__ float loads(const float* p) { return *(const g_f*)p; }
 
+// A lane's winner codes of one step: DPD = D / 16 disparities, 2 bits each.
+template <int D> struct WinWordOf;
+template <> struct WinWordOf<64> { using type = uint8_t; };
+template <> struct WinWordOf<128> { using type = uint16_t; };
+template <> struct WinWordOf<256> { using type = uint32_t; };
+template <int D> using WinWord = typename WinWordOf<D>::type;
+
 // Geometry of one scanline of one direction.
 struct LineGeo {
     int n;            // steps
@@ -99,8 +112,6 @@ struct LineGeo {
     bool active;
     int64_t pix0;     // pixel index of step 0 within the image
     int64_t pstep;    // pixel index increment per step
-    int nseg;         // checkpoints on this line
-    int64_t ckpt0;    // float offset of this line's first checkpoint (per image)
 };
 
 __device__ __forceinline__ LineGeo line_geo(const DsgmArgs& A, int path, int group, int sub) {
@@ -121,8 +132,6 @@ __device__ __forceinline__ LineGeo line_geo(const DsgmArgs& A, int path, int gro
         g.pix0 = (int64_t)(fwd ? 0 : H - 1) * W + line;
         g.pstep = fwd ? W : -W;
     }
-    g.nseg = (g.n + kSeg - 1) / kSeg;
-    g.ckpt0 = (dsgm_ckpt_path_offset(H, W, A.D, path) + (int64_t)line * g.nseg) * A.D;
     return g;
 }
 
@@ -173,228 +182,310 @@ __device__ __forceinline__ int row_argmin(const float (&L)[DPD], float m, int l)
 }
 
 // ------------------------------------------------------------- forward ---
-// S(p) = ((L_0 + L_1) + L_2) + L_3 and its soft-argmin, for one pixel per
-// 16-lane row: s receives S, the row's lane 0 the disparity.
+// Unnormalised softmax weight exp(-(s - mn) / T) = 2^((mn - s) * c) with
+// c = log2(e) / T: one v_exp_f32 after a subtraction and a product.  The
+// forward and the backward use this same form.  The argument is <= 0, and a
+// weight below 2^-126 (a cost more than 87 T above the minimum) contributes
+// nothing to the sums either way.
+__device__ __forceinline__ float soft_weight(float s, float mn, float c) {
+    return __builtin_amdgcn_exp2f((mn - s) * c);
+}
+__device__ __forceinline__ float soft_scale(float invT) { return invT * 1.44269504088896340736f; }
+
+// Soft-argmin of one pixel per 16-lane row: s holds S, every lane of the row
+// receives the disparity, the minimum mn and the normaliser z.
 template <int DPD>
-__device__ __forceinline__ float soft_argmin(const float (&s)[DPD], int l, float invT) {
+__device__ __forceinline__ float soft_argmin(const float (&s)[DPD], int l, float invT, float* mn_out,
+                                             float* z_out) {
     const float mn = row_minf(lane_min(s));
+    const float c = soft_scale(invT);
     float z = 0.f, acc = 0.f;
 #pragma unroll
     for (int k = 0; k < DPD; ++k) {
-        const float e = expf(-(s[k] - mn) * invT);
+        const float e = soft_weight(s[k], mn, c);
         z += e;
         acc += (float)(l * DPD + k) * e;
     }
     z = row_sumf(z);
     acc = row_sumf(acc);
+    *mn_out = mn;
+    *z_out = z;
     return acc / z;
 }
 
-// One direction over a scanline group.  FUSE (direction 2, bottom->top,
-// launched after the other three): instead of storing its volume, every
-// step forms S from the other three volumes and its own L, in the fixed
-// order ((L_0 + L_1) + L_2) + L_3, and writes S and the soft-argmin -- the
-// separate sum pass would read all four volumes back.
-template <int D, bool FUSE>
+// What a forward pass does with each step's L.
+enum FwdMode {
+    kFwdStoreS = 0,   // S = L          (left->right, first of the horizontal pair)
+    kFwdAddS = 1,     // S = S + L      (right->left, second of the pair)
+    kFwdStoreY = 2,   // Y = L          (top->bottom, into the workspace volume)
+    kFwdLast = 3,     // S = (S + Y) + L, then the soft-argmin (bottom->top)
+};
+
+// Loads run ahead of the arithmetic: a scanline step needs its inputs from
+// HBM, and a lone wave (the long horizontal pairs run about one per SIMD)
+// would otherwise wait the full memory latency at every step.  The inputs of
+// step s + kAheadF are issued right after step s has used its slot, into
+// register slots indexed by s % kAheadF (compile-time within the unrolled
+// block).  Steps past the end load the last pixel again and are never used.
+constexpr int kAheadF = 4;
+
+// One direction over a scanline group (forward), writing its checkpoints.
+template <int D, int MODE>
 __device__ __forceinline__ void fwd_lines(const DsgmArgs& A, int b, int path, int group, int lane) {
     constexpr int DPD = D / 16;
+    constexpr bool kPriorS = MODE == kFwdAddS || MODE == kFwdLast;
+    constexpr bool kPriorY = MODE == kFwdLast;
     const int sub = lane >> 4, l = lane & 15;
     const LineGeo g = line_geo(A, path, group, sub);
     const int64_t img = (int64_t)A.H * A.W;
     const float* cost = A.cost + (int64_t)b * img * D + l * DPD;
     const float* p1 = A.p1 + (int64_t)b * img;
     const float* p2 = A.p2 + (int64_t)b * img;
-    float* out = A.paths + ((int64_t)b * 4 + path) * img * D + l * DPD;
-    const float* vol = A.paths + (int64_t)b * 4 * img * D + l * DPD;   // direction r at r * img * D
     float* S = A.S + (int64_t)b * img * D + l * DPD;
+    float* Y = A.vol + (int64_t)b * img * D + l * DPD;
     float* disp = A.disp + (int64_t)b * img;
-    float* ck = A.ckpt + (int64_t)b * dsgm_ckpt_floats_per_image(A.H, A.W, D) + g.ckpt0 + l * DPD;
+    float* stats = A.stats + (int64_t)b * img * 2;
+    const int64_t npix = (int64_t)A.B * img;
+    // saved for the backward: each step's winner codes (this lane's DPD
+    // disparities) and k* of the previous step's L
+    WinWord<D>* wout = reinterpret_cast<WinWord<D>*>(A.wins) + ((int64_t)path * npix + (int64_t)b * img) * 16 + l;
+    uint8_t* kout = A.kstar + (int64_t)path * npix + (int64_t)b * img;
     const float invT = A.inv_temperature;
+    const int n = g.n;
 
-    // the other three directions at pixel q (FUSE)
-    float o0[DPD], o1[DPD], o3[DPD];
-    auto load_others = [&](int64_t q) __attribute__((always_inline)) {
-        loadf<DPD>(vol + q * D, o0);
-        loadf<DPD>(vol + (img + q) * D, o1);
-        loadf<DPD>(vol + (3 * img + q) * D, o3);
+    // inputs of the steps in flight: cost, penalties, and what the pixel's
+    // earlier passes left (S, and Y for the last pass)
+    float qc[kAheadF][DPD], qp1[kAheadF], qp2[kAheadF];
+    float qs[kAheadF][DPD], qy[kAheadF][DPD];
+    auto issue = [&](int slot, int s) __attribute__((always_inline)) {
+        const int64_t q = g.pix0 + (int64_t)min(s, n - 1) * g.pstep;
+        loadf<DPD>(cost + q * D, qc[slot]);
+        qp1[slot] = loads(p1 + q);
+        qp2[slot] = loads(p2 + q);
+        if constexpr (kPriorS) loadf<DPD>(S + q * D, qs[slot]);
+        if constexpr (kPriorY) loadf<DPD>(Y + q * D, qy[slot]);
     };
-    auto emit = [&](int64_t q, const float (&L)[DPD]) __attribute__((always_inline)) {
-        if constexpr (FUSE) {
+    auto emit = [&](int64_t q, const float (&L)[DPD], int slot) __attribute__((always_inline)) {
+        if constexpr (MODE == kFwdLast) {
             float sv[DPD];
 #pragma unroll
-            for (int k = 0; k < DPD; ++k) sv[k] = ((o0[k] + o1[k]) + L[k]) + o3[k];
-            const float d = soft_argmin<DPD>(sv, l, invT);
+            for (int k = 0; k < DPD; ++k) sv[k] = (qs[slot][k] + qy[slot][k]) + L[k];
+            float mn, z;
+            const float d = soft_argmin<DPD>(sv, l, invT, &mn, &z);
             if (g.active) {
                 storef<DPD>(S + q * D, sv);
-                if (l == 0) disp[q] = d;
+                if (l == 0) {
+                    disp[q] = d;
+                    stats[2 * q] = mn;
+                    stats[2 * q + 1] = z;
+                }
             }
+        } else if constexpr (MODE == kFwdAddS) {
+            float sv[DPD];
+#pragma unroll
+            for (int k = 0; k < DPD; ++k) sv[k] = qs[slot][k] + L[k];
+            if (g.active) storef<DPD>(S + q * D, sv);
         } else {
-            if (g.active) storef<DPD>(out + q * D, L);
+            if (g.active) storef<DPD>((MODE == kFwdStoreS ? S : Y) + q * D, L);
         }
     };
 
-    float L[DPD];
-    loadf<DPD>(cost + g.pix0 * D, L);
-    if constexpr (FUSE) load_others(g.pix0);
-    emit(g.pix0, L);
-    if (g.active) storef<DPD>(ck, L);
-    float m = row_minf(lane_min(L));
-    int64_t pix = g.pix0;
-    float c[DPD];
-    if (g.n > 1) loadf<DPD>(cost + (pix + g.pstep) * D, c);
-    for (int s = 1; s < g.n; ++s) {
-        pix += g.pstep;
-        float cur[DPD];
 #pragma unroll
-        for (int k = 0; k < DPD; ++k) cur[k] = c[k];
-        const int64_t nxt = (s + 1 < g.n) ? pix + g.pstep : pix;
-        loadf<DPD>(cost + nxt * D, c);
-        if constexpr (FUSE) load_others(pix);
-        (void)fwd_step<DPD>(L, cur, m, loads(p1 + pix), loads(p2 + pix));
-        m = row_minf(lane_min(L));
-        emit(pix, L);
-        if (g.active && (s & (kSeg - 1)) == 0) storef<DPD>(ck + (int64_t)(s / kSeg) * D, L);
+    for (int u = 0; u < kAheadF; ++u) issue(u, u);
+    float L[DPD];
+    float m = 0.f;
+    for (int s0 = 0; s0 < n; s0 += kAheadF) {
+#pragma unroll
+        for (int u = 0; u < kAheadF; ++u) {
+            const int s = s0 + u;
+            if (s < n) {
+                const int64_t pix = g.pix0 + (int64_t)s * g.pstep;
+                if (s == 0) {
+#pragma unroll
+                    for (int k = 0; k < DPD; ++k) L[k] = qc[u][k];
+                } else {
+                    const int ks = row_argmin<DPD>(L, m, l);
+                    const uint32_t w = fwd_step<DPD>(L, qc[u], m, qp1[u], qp2[u]);
+                    if (g.active) {
+                        wout[pix * 16] = (WinWord<D>)w;
+                        if (l == 0) kout[pix] = (uint8_t)ks;
+                    }
+                }
+                m = row_minf(lane_min(L));
+                emit(pix, L, u);
+            }
+            issue(u, s + kAheadF);
+        }
     }
 }
 
-// Items of a launch over directions 1, 3, 0 (LAST = false) or over
-// direction 2 alone (LAST = true), image by image.
-template <bool LAST>
-__device__ __forceinline__ void item_of(const DsgmArgs& A, int item, int* b, int* path, int* group) {
+// Items of the first launch (forward or backward): the horizontal pairs of
+// every image (4 rows each, left->right then right->left in one wave, so the
+// second accumulates onto the first without another writer), then the
+// top->bottom groups (4 columns each).  The long pairs are dispatched first.
+// The second launch is the bottom->top groups.  Column groups are spread so
+// that each XCD walks a contiguous run of them: the per-pixel maps (penalties,
+// statistics, disparity, gradients) of 32 neighbouring columns share a
+// 128-byte line, which is then fetched into one L2 instead of eight.
+__device__ __forceinline__ void first_item(const DsgmArgs& A, int item, int* b, bool* pair, int* group) {
     const int nh = (A.H + 3) / 4, nv = (A.W + 3) / 4;
-    if constexpr (LAST) {
-        *b = item / nv;
-        *path = 2;
-        *group = item - *b * nv;
-        return;
-    }
-    const int per = 2 * nh + nv;
-    *b = item / per;
-    const int local = item - *b * per;
-    if (local < nh) { *path = 1; *group = local; }
-    else if (local < 2 * nh) { *path = 3; *group = local - nh; }
-    else { *path = 0; *group = local - 2 * nh; }
+    const int hp = A.B * nh;
+    *pair = item < hp;
+    const int local = *pair ? item : dev::xcd_item(item - hp, A.B * nv);
+    const int per = *pair ? nh : nv;
+    *b = local / per;
+    *group = local - *b * per;
 }
 
-template <int D, bool FUSE>
-__global__ __launch_bounds__(64) void dsgm_forward_kernel(DsgmArgs A) {
-    int b, path, group;
-    item_of<FUSE>(A, blockIdx.x, &b, &path, &group);
-    fwd_lines<D, FUSE>(A, b, path, group, threadIdx.x);
+template <int D>
+__global__ __launch_bounds__(64) void dsgm_forward_first_kernel(DsgmArgs A) {
+    int b, group;
+    bool pair;
+    first_item(A, blockIdx.x, &b, &pair, &group);
+    if (pair) {
+        fwd_lines<D, kFwdStoreS>(A, b, 1, group, threadIdx.x);
+        fwd_lines<D, kFwdAddS>(A, b, 3, group, threadIdx.x);
+    } else {
+        fwd_lines<D, kFwdStoreY>(A, b, 0, group, threadIdx.x);
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void dsgm_forward_last_kernel(DsgmArgs A) {
+    const int nv = (A.W + 3) / 4;
+    const int item = dev::xcd_item(blockIdx.x, gridDim.x);
+    const int b = item / nv, group = item - b * nv;
+    fwd_lines<D, kFwdLast>(A, b, 2, group, threadIdx.x);
 }
 
 // ------------------------------------------------------------ backward ---
-// gS(p, d) = gdisp(p) * (-1/T) * prob(p, d) * (d - disp(p))
-template <int D>
-__global__ __launch_bounds__(256) void dsgm_grad_s_kernel(DsgmArgs A) {
-    constexpr int DPD = D / 16;
-    const int64_t img = (int64_t)A.H * A.W;
-    const int64_t p = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-    const int l = threadIdx.x & 15;
-    const bool ok = p < (int64_t)A.B * img;
-    const int64_t pc = ok ? p : 0;
-    float s[DPD];
-    loadf<DPD>(A.S + pc * D + l * DPD, s);
-    const float mn = row_minf(lane_min(s));
-    const float invT = A.inv_temperature;
-    float e[DPD];
-    float z = 0.f;
-#pragma unroll
-    for (int k = 0; k < DPD; ++k) {
-        e[k] = expf(-(s[k] - mn) * invT);
-        z += e[k];
-    }
-    z = row_sumf(z);
-    const float disp = A.disp[pc];
-    const float gd = A.grad_disp[pc];
-    float g[DPD];
+// gS(p, d) = gdisp(p) * (-1/T) * prob(p, d) * (d - disp(p)) of one pixel,
+// from S and the forward's minimum mn and normaliser z of that pixel.
+template <int DPD>
+__device__ __forceinline__ void grad_s_from(const float (&s)[DPD], float mn, float z, float gd,
+                                            float dq, float invT, int l, float (&g)[DPD]) {
+    const float coef = gd * (-invT) / z;
+    const float c = soft_scale(invT);
 #pragma unroll
     for (int k = 0; k < DPD; ++k)
-        g[k] = gd * (-invT) * (e[k] / z) * ((float)(l * DPD + k) - disp);
-    if (ok) storef<DPD>(A.grad_s + pc * D + l * DPD, g);
+        g[k] = coef * soft_weight(s[k], mn, c) * ((float)(l * DPD + k) - dq);
+}
+template <int DPD>
+__device__ __forceinline__ void grad_s_at(const DsgmArgs& A, const float* Sb, const float* stats,
+                                          const float* disp, const float* gdisp, int64_t q, int l,
+                                          float (&g)[DPD]) {
+    float s[DPD];
+    loadf<DPD>(Sb + q * A.D, s);
+    grad_s_from<DPD>(s, loads(stats + 2 * q), loads(stats + 2 * q + 1), loads(gdisp + q),
+                     loads(disp + q), A.inv_temperature, l, g);
 }
 
-// One direction's reverse recurrence over a scanline group.  FUSE
-// (direction 2, launched after the other three): instead of storing its
-// gradients, every pixel adds them to the other three directions' in the
-// fixed order ((g_0 + g_1) + g_2) + g_3 and writes dC, dP1 and dP2.
-template <int D, bool FUSE>
+// What a backward pass does with each pixel's gradients of its direction.
+enum BwdMode {
+    kBwdStore = 0,    // dC = G, dP1 = d1, dP2 = d2            (left->right)
+    kBwdAdd = 1,      // dC += G, dP1 += d1, dP2 += d2         (right->left)
+    kBwdStoreY = 2,   // workspace: gY = G, gY1 = d1, gY2 = d2 (top->bottom)
+    kBwdLast = 3,     // dC = (dC + gY) + G, likewise dP1, dP2 (bottom->top)
+};
+
+// Loads ahead in the backward: what transition s needs (its winner codes
+// and k*, S and the soft-argmin statistics of the pixel before it, and, for
+// the passes that accumulate, the gradients the pixel already holds) is
+// issued kAheadV transitions earlier, into register slots indexed by the
+// position within an unrolled block.  Loads for transitions below 1 read
+// transition 1 again and are not used.
+constexpr int kAheadV = 4;
+
+// One direction's reverse recurrence over a scanline group.
+template <int D, int MODE>
 __device__ __forceinline__ void bwd_lines(const DsgmArgs& A, int b, int path, int group, int lane) {
     constexpr int DPD = D / 16;
+    constexpr int V = kAheadV;
+    constexpr bool kPriorC = MODE == kBwdAdd || MODE == kBwdLast;
+    constexpr bool kPriorY = MODE == kBwdLast;
     const int sub = lane >> 4, l = lane & 15;
     const LineGeo g = line_geo(A, path, group, sub);
     const int64_t img = (int64_t)A.H * A.W;
-    const float* cost = A.cost + (int64_t)b * img * D + l * DPD;
-    const float* p1 = A.p1 + (int64_t)b * img;
-    const float* p2 = A.p2 + (int64_t)b * img;
-    const float* gs = A.grad_s + (int64_t)b * img * D + l * DPD;
-    const float* ck = A.ckpt + (int64_t)b * dsgm_ckpt_floats_per_image(A.H, A.W, D) + g.ckpt0 + l * DPD;
-    float* gc = A.grad_paths + ((int64_t)b * 4 + path) * img * D + l * DPD;
-    float* gp1 = A.grad_p1_paths + ((int64_t)b * 4 + path) * img;
-    float* gp2 = A.grad_p2_paths + ((int64_t)b * 4 + path) * img;
-    const float* gv = A.grad_paths + (int64_t)b * 4 * img * D + l * DPD;   // direction r at r * img * D
-    const float* a1v = A.grad_p1_paths + (int64_t)b * 4 * img;
-    const float* a2v = A.grad_p2_paths + (int64_t)b * 4 * img;
+    const int64_t npix = (int64_t)A.B * img;
+    const float* Sb = A.S + (int64_t)b * img * D + l * DPD;
+    const float* stats = A.stats + (int64_t)b * img * 2;
+    const float* dsp = A.disp + (int64_t)b * img;
+    const float* gdisp = A.grad_disp + (int64_t)b * img;
+    const WinWord<D>* win = reinterpret_cast<const WinWord<D>*>(A.wins) +
+                            ((int64_t)path * npix + (int64_t)b * img) * 16 + l;
+    const uint8_t* kin = A.kstar + (int64_t)path * npix + (int64_t)b * img;
     float* out_c = A.grad_cost + (int64_t)b * img * D + l * DPD;
     float* out_p1 = A.grad_p1 + (int64_t)b * img;
     float* out_p2 = A.grad_p2 + (int64_t)b * img;
+    float* gy = A.vol + (int64_t)b * img * D + l * DPD;
+    float* gy1 = A.vol_p1 + (int64_t)b * img;
+    float* gy2 = A.vol_p2 + (int64_t)b * img;
     const int n = g.n;
-    // this direction's gradients at pixel q: stored, or summed with the others
-    auto emit = [&](int64_t q, const float (&G)[DPD], float d1, float d2) __attribute__((always_inline)) {
+    const float invT = A.inv_temperature;
+    auto pix_at = [&](int s) __attribute__((always_inline)) { return g.pix0 + (int64_t)s * g.pstep; };
+
+    // inputs of the transitions in flight
+    uint32_t vw[V], vk[V];
+    float vs[V][DPD], vmn[V], vz[V], vgd[V], vdq[V];
+    float vo[V][DPD], vo1[V], vo2[V];
+    float vy[V][DPD], vy1[V], vy2[V];
+    auto issue = [&](int slot, int s) __attribute__((always_inline)) {
+        if (n < 2) return;   // no transitions
+        const int64_t q = pix_at(max(s, 1)), qp = q - g.pstep;
+        vw[slot] = win[q * 16];
+        vk[slot] = kin[q];
+        loadf<DPD>(Sb + qp * D, vs[slot]);
+        vmn[slot] = loads(stats + 2 * qp);
+        vz[slot] = loads(stats + 2 * qp + 1);
+        vgd[slot] = loads(gdisp + qp);
+        vdq[slot] = loads(dsp + qp);
+        if constexpr (kPriorC) {
+            loadf<DPD>(out_c + q * D, vo[slot]);
+            vo1[slot] = loads(out_p1 + q);
+            vo2[slot] = loads(out_p2 + q);
+        }
+        if constexpr (kPriorY) {
+            loadf<DPD>(gy + q * D, vy[slot]);
+            vy1[slot] = loads(gy1 + q);
+            vy2[slot] = loads(gy2 + q);
+        }
+    };
+    // writes this direction's gradients of one pixel; o, o1, o2 (and y, y1,
+    // y2) are what the pixel held before, for the passes that accumulate
+    auto emit = [&](int64_t q, const float (&G)[DPD], float d1, float d2, const float (&o)[DPD],
+                    float o1, float o2, const float (&y)[DPD], float y1, float y2)
+        __attribute__((always_inline)) {
         if (!g.active) return;
-        if constexpr (FUSE) {
-            float o0[DPD], o1[DPD], o3[DPD], sv[DPD];
-            loadf<DPD>(gv + q * D, o0);
-            loadf<DPD>(gv + (img + q) * D, o1);
-            loadf<DPD>(gv + (3 * img + q) * D, o3);
-#pragma unroll
-            for (int k = 0; k < DPD; ++k) sv[k] = ((o0[k] + o1[k]) + G[k]) + o3[k];
-            storef<DPD>(out_c + q * D, sv);
+        if constexpr (MODE == kBwdStore || MODE == kBwdStoreY) {
+            float* oc = MODE == kBwdStore ? out_c : gy;
+            storef<DPD>(oc + q * D, G);
             if (l == 0) {
-                out_p1[q] = ((a1v[q] + a1v[img + q]) + d1) + a1v[3 * img + q];
-                out_p2[q] = ((a2v[q] + a2v[img + q]) + d2) + a2v[3 * img + q];
+                (MODE == kBwdStore ? out_p1 : gy1)[q] = d1;
+                (MODE == kBwdStore ? out_p2 : gy2)[q] = d2;
             }
         } else {
-            storef<DPD>(gc + q * D, G);
-            if (l == 0) { gp1[q] = d1; gp2[q] = d2; }
+            float v[DPD];
+#pragma unroll
+            for (int k = 0; k < DPD; ++k) v[k] = (MODE == kBwdLast) ? (o[k] + y[k]) + G[k] : o[k] + G[k];
+            storef<DPD>(out_c + q * D, v);
+            if (l == 0) {
+                out_p1[q] = (MODE == kBwdLast) ? (o1 + y1) + d1 : o1 + d1;
+                out_p2[q] = (MODE == kBwdLast) ? (o2 + y2) + d2 : o2 + d2;
+            }
         }
     };
 
     float G[DPD];
-    loadf<DPD>(gs + (g.pix0 + (int64_t)(n - 1) * g.pstep) * D, G);
-    for (int j = g.nseg - 1; j >= 0; --j) {
-        const int s0 = j * kSeg;
-        const int s1 = min(s0 + kSeg, n - 1);     // last transition of this segment
-        // recompute L(s0 .. s1-1), recording the transitions s0+1 .. s1
-        uint32_t wins[kSeg];
-        uint32_t kst[kSeg / 4];
+    grad_s_at<DPD>(A, Sb, stats, dsp, gdisp, pix_at(n - 1), l, G);
 #pragma unroll
-        for (int t = 0; t < kSeg / 4; ++t) kst[t] = 0u;
-        float L[DPD];
-        loadf<DPD>(ck + (int64_t)j * D, L);
-        float m = row_minf(lane_min(L));
+    for (int u = 0; u < V; ++u) issue(u, n - 1 - u);
+    for (int s0 = n - 1; s0 >= 1; s0 -= V) {
 #pragma unroll
-        for (int t = 0; t < kSeg; ++t) {
-            const int s = s0 + 1 + t;
-            if (s <= s1) {
-                const int ks = row_argmin<DPD>(L, m, l);
-                kst[t >> 2] |= (uint32_t)ks << (8 * (t & 3));
-                const int64_t pix = g.pix0 + (int64_t)s * g.pstep;
-                float c[DPD];
-                loadf<DPD>(cost + pix * D, c);
-                wins[t] = fwd_step<DPD>(L, c, m, loads(p1 + pix), loads(p2 + pix));
-                m = row_minf(lane_min(L));
-            } else {
-                wins[t] = 0u;
-            }
-        }
-        // reverse sweep over the transitions s1 .. s0+1
-#pragma unroll
-        for (int t = kSeg - 1; t >= 0; --t) {
-            const int s = s0 + 1 + t;
-            if (s <= s1) {
-                const int64_t pix = g.pix0 + (int64_t)s * g.pstep;
-                const uint32_t w = wins[t];
+        for (int u = 0; u < V; ++u) {
+            const int s = s0 - u;    // the transition s-1 -> s, in slot u
+            if (s >= 1) {
+                const int64_t pix = pix_at(s);
+                const uint32_t w = vw[u];
                 float all = 0.f, s3 = 0.f, s12 = 0.f;
                 float A1[DPD], A2[DPD];
 #pragma unroll
@@ -409,36 +500,64 @@ __device__ __forceinline__ void bwd_lines(const DsgmArgs& A, int b, int path, in
                 all = row_sumf(all);
                 s3 = row_sumf(s3);
                 s12 = row_sumf(s12);
-                emit(pix, G, s12, s3);
+                emit(pix, G, s12, s3, vo[u], vo1[u], vo2[u], vy[u], vy1[u], vy2[u]);
                 const float a1_next = dppf<kShl1>(0.f, A1[0]);        // lane+1, d = lane end + 1
                 const float a2_prev = dppf<kShr1>(0.f, A2[DPD - 1]);  // lane-1, d = lane start - 1
-                const int ks = (int)((kst[t >> 2] >> (8 * (t & 3))) & 0xFFu);
+                const int ks = (int)vk[u];
                 float gprev[DPD];
-                loadf<DPD>(gs + (pix - g.pstep) * D, gprev);
+                grad_s_from<DPD>(vs[u], vmn[u], vz[u], vgd[u], vdq[u], invT, l, gprev);
                 const float jump = s3 - all;
 #pragma unroll
                 for (int k = 0; k < DPD; ++k) {
                     const uint32_t wk = (w >> (2 * k)) & 3u;
-                    float v = gprev[k];
-                    v += (wk == 0u) ? G[k] : 0.f;
-                    v += (k == DPD - 1) ? a1_next : A1[k + 1];
-                    v += (k == 0) ? a2_prev : A2[k - 1];
-                    v += (l * DPD + k == ks) ? jump : 0.f;
-                    gprev[k] = v;
+                    float x = gprev[k];
+                    x += (wk == 0u) ? G[k] : 0.f;
+                    x += (k == DPD - 1) ? a1_next : A1[k + 1];
+                    x += (k == 0) ? a2_prev : A2[k - 1];
+                    x += (l * DPD + k == ks) ? jump : 0.f;
+                    gprev[k] = x;
                 }
 #pragma unroll
                 for (int k = 0; k < DPD; ++k) G[k] = gprev[k];
             }
+            issue(u, s - V);
         }
     }
-    emit(g.pix0, G, 0.f, 0.f);   // no penalty at the first pixel
+    // the first pixel (no penalty terms)
+    float o0[DPD], y0[DPD], o01 = 0.f, o02 = 0.f, y01 = 0.f, y02 = 0.f;
+    const int64_t q0 = g.pix0;
+    if constexpr (kPriorC) {
+        loadf<DPD>(out_c + q0 * D, o0);
+        o01 = loads(out_p1 + q0);
+        o02 = loads(out_p2 + q0);
+    }
+    if constexpr (kPriorY) {
+        loadf<DPD>(gy + q0 * D, y0);
+        y01 = loads(gy1 + q0);
+        y02 = loads(gy2 + q0);
+    }
+    emit(q0, G, 0.f, 0.f, o0, o01, o02, y0, y01, y02);
 }
 
-template <int D, bool FUSE>
-__global__ __launch_bounds__(64) void dsgm_backward_kernel(DsgmArgs A) {
-    int b, path, group;
-    item_of<FUSE>(A, blockIdx.x, &b, &path, &group);
-    bwd_lines<D, FUSE>(A, b, path, group, threadIdx.x);
+template <int D>
+__global__ __launch_bounds__(64) void dsgm_backward_first_kernel(DsgmArgs A) {
+    int b, group;
+    bool pair;
+    first_item(A, blockIdx.x, &b, &pair, &group);
+    if (pair) {
+        bwd_lines<D, kBwdStore>(A, b, 1, group, threadIdx.x);
+        bwd_lines<D, kBwdAdd>(A, b, 3, group, threadIdx.x);
+    } else {
+        bwd_lines<D, kBwdStoreY>(A, b, 0, group, threadIdx.x);
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void dsgm_backward_last_kernel(DsgmArgs A) {
+    const int nv = (A.W + 3) / 4;
+    const int item = dev::xcd_item(blockIdx.x, gridDim.x);
+    const int b = item / nv, group = item - b * nv;
+    bwd_lines<D, kBwdLast>(A, b, 2, group, threadIdx.x);
 }
 
 }  // namespace
@@ -452,41 +571,29 @@ __global__ __launch_bounds__(64) void dsgm_backward_kernel(DsgmArgs A) {
             default: return hipErrorInvalidValue;                                       \
         }                                                                               \
     } while (0)
-#define FSGM_DSGM_DISPATCH2(KER, FUSE, grid, block, A, s)                                \
-    do {                                                                                \
-        switch ((A).D) {                                                                \
-            case 64: hipLaunchKernelGGL((KER<64, FUSE>), grid, block, 0, s, A); break;  \
-            case 128: hipLaunchKernelGGL((KER<128, FUSE>), grid, block, 0, s, A); break; \
-            case 256: hipLaunchKernelGGL((KER<256, FUSE>), grid, block, 0, s, A); break; \
-            default: return hipErrorInvalidValue;                                       \
-        }                                                                               \
-    } while (0)
 
-// scanline groups of directions 1, 3 and 0, and of direction 2
-static int dsgm_items3(const DsgmArgs& A) { return A.B * (2 * ((A.H + 3) / 4) + (A.W + 3) / 4); }
-static int dsgm_items1(const DsgmArgs& A) { return A.B * ((A.W + 3) / 4); }
+// first launch: the horizontal pairs and the top->bottom groups of every
+// image; second launch: the bottom->top groups
+static unsigned dsgm_items_first(const DsgmArgs& A) {
+    return (unsigned)(A.B * ((A.H + 3) / 4 + (A.W + 3) / 4));
+}
+static unsigned dsgm_items_last(const DsgmArgs& A) { return (unsigned)(A.B * ((A.W + 3) / 4)); }
 
-// directions 1, 3, 0 store their volumes; direction 2 then sums the four and
-// takes the soft-argmin
 hipError_t dsgm_launch_forward(const DsgmArgs& A, hipStream_t s) {
     const dim3 b1(64);
-    FSGM_DSGM_DISPATCH2(dsgm_forward_kernel, false, dim3(dsgm_items3(A)), b1, A, s);
+    FSGM_DSGM_DISPATCH(dsgm_forward_first_kernel, dim3(dsgm_items_first(A)), b1, A, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    FSGM_DSGM_DISPATCH2(dsgm_forward_kernel, true, dim3(dsgm_items1(A)), b1, A, s);
+    FSGM_DSGM_DISPATCH(dsgm_forward_last_kernel, dim3(dsgm_items_last(A)), b1, A, s);
     return hipGetLastError();
 }
 
 hipError_t dsgm_launch_backward(const DsgmArgs& A, hipStream_t s) {
-    const int64_t npix = (int64_t)A.B * A.H * A.W;
-    const dim3 g0((unsigned)((npix + 15) / 16)), b0(256);
-    FSGM_DSGM_DISPATCH(dsgm_grad_s_kernel, g0, b0, A, s);
+    const dim3 b1(64);
+    FSGM_DSGM_DISPATCH(dsgm_backward_first_kernel, dim3(dsgm_items_first(A)), b1, A, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const dim3 b1(64);
-    FSGM_DSGM_DISPATCH2(dsgm_backward_kernel, false, dim3(dsgm_items3(A)), b1, A, s);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    FSGM_DSGM_DISPATCH2(dsgm_backward_kernel, true, dim3(dsgm_items1(A)), b1, A, s);
+    FSGM_DSGM_DISPATCH(dsgm_backward_last_kernel, dim3(dsgm_items_last(A)), b1, A, s);
     return hipGetLastError();
 }
 

@@ -6,9 +6,10 @@
 The forward and the hand-written backward are HIP kernels in libfsgm.so
 (sgm_dsgm_forward / sgm_dsgm_backward); this module only allocates their
 buffers with torch's caching allocator and passes the current stream.  The
-forward saves S (the 4-direction sum) and the per-direction checkpoints
-taken every 32 scanline steps; the full per-direction volumes live only in a
-scratch buffer during each kernel call.  The torch-free C++ API never
+forward saves S (the 4-direction sum) and, per direction, the winning term of
+every cell (2 bits) and each pixel's argmin byte, about a quarter of one
+volume; no per-direction volume is kept.  Each call's transient workspace is
+one direction volume plus two per-pixel planes.  The torch-free C++ API never
 depends on this module.
 """
 from __future__ import annotations

@@ -243,10 +243,12 @@ int sgm_batch_use_graph(sgm_batch* b, int32_t on);
  * image with the textbook boundary (no d-1 term at d = 0, no d+1 term at
  * d = D-1).  D must be 64, 128 or 256.  All buffers are device memory owned
  * by the caller; sizes in floats come from the two query functions.  The
- * forward stores S (needed by the backward), per-direction checkpoints every
- * 32 scanline steps and two floats per pixel in d_ckpt; no per-direction
- * volume is kept.  The transient workspace (d_work, shared by forward and
- * backward) is one direction volume plus two per-pixel planes. */
+ * forward stores S (needed by the backward) and, in d_ckpt, what the
+ * backward needs of each direction: the winning term of every cell (2 bits)
+ * and the argmin of the previous step per pixel (1 byte), plus two floats
+ * per pixel; about a quarter of one volume, and no per-direction volume.
+ * The transient workspace (d_work, shared by forward and backward) is one
+ * direction volume plus two per-pixel planes. */
 size_t sgm_dsgm_checkpoint_floats(int32_t B, int32_t H, int32_t W, int32_t D);
 size_t sgm_dsgm_workspace_floats(int32_t B, int32_t H, int32_t W, int32_t D);
 int sgm_dsgm_forward(const float* d_cost, const float* d_p1, const float* d_p2, int32_t B,
@@ -254,7 +256,8 @@ int sgm_dsgm_forward(const float* d_cost, const float* d_p1, const float* d_p2, 
                      float* d_disp, float* d_ckpt, float* d_work, void* stream);
 /* Gradients of a scalar loss given dLoss/d disp.  dLoss/dS is formed from S
  * where it is used; d_grad_cost, d_grad_p1 and d_grad_p2 are built in place
- * over the directions. */
+ * over the directions.  The recurrences are reversed from what the forward
+ * saved in d_ckpt: d_cost, d_p1 and d_p2 are checked but not read. */
 int sgm_dsgm_backward(const float* d_cost, const float* d_p1, const float* d_p2,
                       const float* d_S, const float* d_disp, const float* d_ckpt,
                       const float* d_grad_disp, int32_t B, int32_t H, int32_t W, int32_t D,

@@ -238,3 +238,19 @@ def test_roi_from_disparity_host_scan():
     d[5, 2] = 3.0
     roi = StereoCuda.GetRoiFromDispMap(d)
     assert (roi.x, roi.y, roi.w, roi.h) == (2, 3, 8, 5)
+
+
+@pytest.mark.parametrize("B,H,W,D", [(8, 375, 1242, 128), (1, 33, 40, 64), (2, 20, 64, 256)])
+def test_dsgm_memory_is_bounded(B, H, W, D):
+    """The differentiable SGM keeps no per-direction volume.  What the forward
+    saves for the backward is two floats per pixel, the winner codes (2 bits
+    per cell and direction: one byte per cell) and one argmin byte per pixel
+    and direction, about a quarter of one FP32 volume; the transient
+    workspace is one direction volume plus two per-pixel planes."""
+    L = _lib.lib()
+    npix = B * H * W
+    saved = L.sgm_dsgm_checkpoint_floats(B, H, W, D)
+    assert saved == -(-(npix * 8 + npix * D + npix * 4) // 4)
+    assert 4 * saved < 0.3 * 4 * npix * D          # under 0.3 of a volume
+    assert L.sgm_dsgm_workspace_floats(B, H, W, D) == npix * D + 2 * npix
+    assert L.sgm_dsgm_workspace_floats(B, H, W, 96) == 0   # D must be 64, 128 or 256
