@@ -356,6 +356,17 @@ int run_pyramid(sgm_pyramid* p, hipStream_t s) {
 
 extern "C" {
 
+int sgm_set_window_form(int32_t form) {
+    if (form != SGM_WINDOW_AUTO && form != SGM_WINDOW_U32 && form != SGM_WINDOW_PAIRS &&
+        form != SGM_WINDOW_QUADS)
+        return set_global_error(SGM_E_INVALID_ARG,
+                                "window form must be SGM_WINDOW_AUTO, _U32, _PAIRS or _QUADS");
+    fsgm::set_window_form(form);
+    return SGM_OK;
+}
+
+int32_t sgm_get_window_form(void) { return fsgm::window_form(); }
+
 int sgm_pyramid_match_device(sgm_pyramid* p, const uint8_t* d_left, const uint8_t* d_right,
                              float* d_disp, void* stream) {
     if (!p || !d_left || !d_right || !d_disp)

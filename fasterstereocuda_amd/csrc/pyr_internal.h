@@ -42,6 +42,9 @@ hipError_t pyr_launch_init_next(const PyrInit& a, hipStream_t s);
 hipError_t pyr_launch_dilation(const float* in, float* out, int w, int h, int wnd, hipStream_t s);
 hipError_t pyr_launch_cost_init(const PyrLevel& a, hipStream_t s);
 hipError_t pyr_launch_aggregate_init(const PyrLevel& a, hipStream_t s);
+// process-wide SGM_WINDOW_* form of the window aggregation
+int window_form();
+void set_window_form(int form);
 hipError_t pyr_launch_wta_init(const PyrLevel& a, hipStream_t s);
 
 // plan upload shared by the plain, batched and pyramid paths (sgm_capi.hip)

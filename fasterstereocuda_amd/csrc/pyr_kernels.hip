@@ -11,6 +11,8 @@
 // lane shift, fetched with ds_bpermute.  Arithmetic is uint32 with the
 // reference's wrap-around after INVALID base pixels.
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 
 #include "pyr_internal.h"
 
@@ -256,6 +258,504 @@ __global__ __launch_bounds__(64) void agg_init_kernel(PyrLevel a) {
     else agg_init_lines<LR, false>(a, dir, group, threadIdx.x, lds);
 }
 
+// The same aggregation with two disparities per lane as packed u16 pairs:
+// LR / 2 lanes per scanline, SPW = 128 / LR scanlines per wave, and every
+// VALU instruction of the step works on two cells.  The window shift goes
+// through LDS.  Each step writes the scanline's L + P1 into a row of
+// overlapping pairs, C[e] = (L'[e - PAD], L'[e - PAD + 1]) + P1, padded on
+// both sides with 127.  A lane then reads the four neighbour terms its two
+// disparities need, L'[2k + b - b' - 1 .. 2k + b - b' + 2] + P1, as the two
+// dwords C[e0] and C[e0 + 2]: dword-aligned whatever the parity of b - b'.
+// (Reading the four u16 of a plain row is a misaligned 8-byte LDS access,
+// and its stalls took most of the kernel's time.)  A pad is exactly the 127
+// of a neighbour outside the window, so no range test remains.  b - b' is
+// clamped to +-(LR + 1), past which every term is a pad, and, separately, to
+// [-254, 256] to index a table of max(P1, P2 / max(1, |b' - b + 1|)).
+//
+// Exact when P1 <= 127 and P2 <= 254: the true (unbounded) values of the
+// recurrence then lie in [-382, 763] (l <= c + P2 <= 509, and l >= 127 - m
+// after a neighbour outside the window), so the minima of their u16
+// representatives pick the same entries as those of their u32
+// representatives, and the stored low bytes agree.  Other penalties take
+// agg_init_kernel.
+typedef unsigned short pk16 __attribute__((ext_vector_type(2)));
+
+constexpr int kPkDeltaLo = -254, kPkDeltaHi = 256;   // table range of b - b'
+// Blocks of cost loads in flight per wave.  A block is LR / 2 steps; its
+// loads are issued kPkAhead blocks before they are staged, so the HBM
+// latency (1-2 us under load) is spread over that many blocks of steps.
+constexpr int kPkAhead = 4;
+
+template <int LR>
+struct AggInitPkLds {
+    static constexpr int SPW = 128 / LR;
+    static constexpr int PAD = LR + 2;               // even: the lanes' pair stores stay 8-byte aligned
+    static constexpr int ROW = PAD + 2 * LR + 2;     // even; reads reach PAD + 2 LR
+    uint32_t cost[SPW][LR / 2][LR / 2];      // [line][pair k][step]: the pair's costs as u16
+    uint8_t out[64 * LR];                    // [line][step][t]
+    int16_t init[64];                        // [line][step]
+    uint32_t pen2[kPkDeltaHi - kPkDeltaLo + 1];   // by b - b', both halves
+    uint32_t win[SPW][ROW];                  // overlapping pairs, see above
+};
+
+__device__ __forceinline__ uint32_t pk_bits(pk16 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ pk16 pk_of(uint32_t v) { return __builtin_bit_cast(pk16, v); }
+
+// minimum over each aligned group of LR / 2 lanes of both halves; the result
+// holds it in both halves
+template <int LR>
+__device__ __forceinline__ pk16 pk_group_min(pk16 v) {
+    v = __builtin_elementwise_min(v, v.yx);
+    uint32_t m = pk_bits(v);                 // equal halves: u32 order = u16 order
+    m = min(m, dppu<0xB1>(m));               // quad_perm [1, 0, 3, 2]
+    m = min(m, dppu<0x4E>(m));               // quad_perm [2, 3, 0, 1]
+    m = min(m, dppu<0x141>(m));              // row_half_mirror
+    if constexpr (LR == 32) m = min(m, dppu<0x140>(m));   // row_mirror
+    return pk_of(m);
+}
+
+template <int LR>
+__device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, int group, int lane,
+                                                  AggInitPkLds<LR>& S) {
+    constexpr int LPL = LR / 2, SPW = 128 / LR, K = LPL;
+    constexpr int PAD = AggInitPkLds<LR>::PAD, ROW = AggInitPkLds<LR>::ROW;
+    constexpr int NV = LR / 16;
+    constexpr uint32_t kPadPair = 127u * 0x10001u;
+    const int sub = lane / LPL, k = lane % LPL;      // recurrence: line, disparity pair
+    const int lsub = lane / K, lstep = lane % K;     // loads / stores: line, step
+    const bool horiz = (dir == 1 || dir == 3);
+    const bool fwd = (dir == 0 || dir == 1);
+    const int W = a.w, H = a.h;
+    const int nlines = horiz ? H : W;
+    const int n = horiz ? W : H;
+    auto pix_of = [&](int line, int step) -> int64_t {
+        line = min(line, nlines - 1);
+        step = min(step, n - 1);
+        const int pos = fwd ? step : n - 1 - step;
+        return horiz ? (int64_t)line * W + pos : (int64_t)pos * W + line;
+    };
+    const int line = min(group * SPW + sub, nlines - 1);
+    const int lline = group * SPW + lsub;
+    uint8_t* out = (dir == 0) ? a.path[0] : (dir == 1) ? a.path[1] : (dir == 2) ? a.path[2] : a.path[3];
+
+    // a ring of kPkAhead blocks of loads in flight (slot = block mod kPkAhead)
+    uint4 ncs[kPkAhead][NV];
+    int nbs[kPkAhead];
+    auto fetch = [&](int slot, int blk) __attribute__((always_inline)) {
+        const int64_t p = pix_of(lline, blk * K + lstep);
+        const uint4* src = reinterpret_cast<const uint4*>(a.cost + p * LR);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ncs[slot][v] = src[v];
+        nbs[slot] = a.init[p];
+    };
+    // the block in LDS, transposed so that a lane later reads the cost pairs
+    // of all K steps of its own (line, pair) with wide loads: pair j of
+    // this lane's pixel goes to cost[line][j][step], zero-extended to u16
+    auto stage = [&](int slot) __attribute__((always_inline)) {
+        uint32_t wv[LR / 4];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            wv[4 * v] = ncs[slot][v].x; wv[4 * v + 1] = ncs[slot][v].y;
+            wv[4 * v + 2] = ncs[slot][v].z; wv[4 * v + 3] = ncs[slot][v].w;
+        }
+#pragma unroll
+        for (int j = 0; j < LR / 2; ++j)
+            S.cost[lsub][j][lstep] = __builtin_amdgcn_perm(0u, wv[j / 2], (j & 1) ? 0x0c030c02u : 0x0c010c00u);
+        S.init[lane] = (int16_t)nbs[slot];
+    };
+
+    // pads, and the P2 table by offset q = b - b' (divisor |1 - q|)
+    for (int e = lane; e < SPW * ROW; e += 64) (&S.win[0][0])[e] = kPadPair;
+    for (int e = lane; e <= kPkDeltaHi - kPkDeltaLo; e += 64) {
+        const int q = e + kPkDeltaLo;
+        const uint32_t v = (uint32_t)max(a.p1, a.p2 / max(1, abs(1 - q)));
+        S.pen2[e] = v | (v << 16);
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    uint32_t* wrow = &S.win[sub][0];
+    const uint32_t* wlane = wrow + (PAD - 1 + 2 * k);     // + q: the pair before this lane's first term
+    const bool last_k = k == LPL - 1;
+    uint8_t* obase = S.out + sub * K * LR + 2 * k;
+    const pk16 p1v = pk_of((uint32_t)a.p1 * 0x10001u);
+    const pk16 m127 = pk_of(kPadPair);
+    // clamp bounds kept in registers (one v_med3 each per step) instead of
+    // rematerialised each step
+    int qlo = kPkDeltaLo, qhi = kPkDeltaHi, wlo = -(LR + 1);
+    asm volatile("" : "+v"(qlo), "+v"(qhi), "+v"(wlo));
+    const bool first_k = k == 0;
+    // P2 terms of the offsets -1, 0 and 1 (divisors 2, 1 and 1)
+    const pk16 pen2_m1 = pk_of(S.pen2[-1 - kPkDeltaLo]);
+    const pk16 pen2_01 = pk_of(S.pen2[0 - kPkDeltaLo]);
+    // this lane's pairs of L + P1 into the window row: C[PAD + 2k] (its own
+    // pair), C[PAD + 2k + 1] (its high half and the next lane's low half or a
+    // pad) and, from lane 0, C[PAD - 1] (a pad and its low half)
+    auto put = [&](uint32_t own) __attribute__((always_inline)) {
+        uint32_t nxt = dppu<0x101>(own);                  // row_shl:1, lane k + 1
+        nxt = last_k ? kPadPair : nxt;
+        const uint32_t mid = __builtin_amdgcn_alignbit(nxt, own, 16);
+        *reinterpret_cast<uint2*>(wrow + PAD + 2 * k) = make_uint2(own, mid);
+        if (k == 0) wrow[PAD - 1] = (own << 16) | 127u;
+    };
+
+    pk16 L = pk_of(0u), m = pk_of(0u);
+    int last_b = 0;
+    const int nblk = (n + K - 1) / K;
+    auto process = [&](int blk) __attribute__((always_inline)) {
+        __builtin_amdgcn_wave_barrier();
+        // the bases of the block's K steps of this lane's line and this
+        // lane's cost pairs of those steps, in registers: no LDS read on the
+        // step's chain
+        uint32_t bw[K / 2], cwv[K];
+#pragma unroll
+        for (int i = 0; i < K / 8; ++i) {
+            const uint4 v = reinterpret_cast<const uint4*>(&S.init[sub * K])[i];
+            bw[4 * i] = v.x; bw[4 * i + 1] = v.y; bw[4 * i + 2] = v.z; bw[4 * i + 3] = v.w;
+        }
+#pragma unroll
+        for (int i = 0; i < K / 4; ++i) {
+            const uint4 v = reinterpret_cast<const uint4*>(&S.cost[sub][k][0])[i];
+            cwv[4 * i] = v.x; cwv[4 * i + 1] = v.y; cwv[4 * i + 2] = v.z; cwv[4 * i + 3] = v.w;
+        }
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            const int b = (int)(int16_t)(bw[s / 2] >> (16 * (s & 1)));
+            const pk16 c = pk_of(cwv[s]);
+            pk16 o;
+            if (s == 0 && blk == 0) {
+                L = c;
+                m = pk_group_min<LR>(L);
+                // A11: vertical directions seed b' from row 0 whichever end they start at
+                last_b = horiz ? b : a.init[line];
+                o = L;
+            } else {
+                const bool valid = b != kInvalidShort;
+                const int q = b - last_b;
+                const uint32_t own = pk_bits(L + p1v);
+                pk16 bm, bp, pen2;                        // neighbour terms, + P1 already
+                if (__builtin_amdgcn_ballot_w64(valid && (q < -1 || q > 1)) == 0) {
+                    // every line shifts by at most one disparity: the terms are
+                    // this lane's pair and its neighbours' (DPP), no LDS
+                    uint32_t nxt = dppu<0x101>(own);          // row_shl:1, lane k + 1
+                    uint32_t prv = dppu<0x111>(own);          // row_shr:1, lane k - 1
+                    nxt = last_k ? kPadPair : nxt;
+                    prv = first_k ? kPadPair : prv;
+                    const uint32_t lo_mid = __builtin_amdgcn_alignbit(own, prv, 16);   // 2k-1, 2k
+                    const uint32_t hi_mid = __builtin_amdgcn_alignbit(nxt, own, 16);   // 2k+1, 2k+2
+                    const uint32_t mpos = 0u - (uint32_t)(q > 0), mzero = 0u - (uint32_t)(q == 0);
+                    const uint32_t bm0 = (lo_mid & mzero) | (prv & ~mzero);   // q = 0 or -1
+                    const uint32_t bp0 = (hi_mid & mzero) | (own & ~mzero);
+                    bm = pk_of((own & mpos) | (bm0 & ~mpos));
+                    bp = pk_of((nxt & mpos) | (bp0 & ~mpos));
+                    const uint32_t mneg = 0u - (uint32_t)(q < 0);
+                    pen2 = pk_of((pk_bits(pen2_m1) & mneg) | (pk_bits(pen2_01) & ~mneg));
+                } else {
+                    put(own);      // the window row holds L only when a step needs it
+                    const int qt = min(max(q, qlo), qhi);
+                    const int qw = min(max(q, wlo), LR + 1);
+                    uint32_t xm = wlane[qw], xp = wlane[qw + 2], pb = S.pen2[qt - kPkDeltaLo];
+                    // complete these reads inside the branch, so that no LDS read
+                    // is pending where the two paths meet (the fast path would
+                    // otherwise wait for every LDS operation, the output stores
+                    // included)
+                    asm volatile("" : "+v"(xm), "+v"(xp), "+v"(pb));
+                    bm = pk_of(xm);
+                    bp = pk_of(xp);
+                    pen2 = pk_of(pb);
+                }
+                const pk16 v = __builtin_elementwise_min(__builtin_elementwise_min(L, bm),
+                                                         __builtin_elementwise_min(bp, m + pen2));
+                const pk16 l = c + (v - m);
+                const pk16 mn = pk_group_min<LR>(l);
+                // selects as bit operations (v_bfi), so the step stays one block
+                const uint32_t vm = 0u - (uint32_t)valid;
+                o = pk_of(pk_bits(l) & vm);
+                L = pk_of((pk_bits(l) & vm) | (pk_bits(L) & ~vm));
+                m = pk_of((pk_bits(mn) & vm) | (pk_bits(m127) & ~vm));
+                last_b = (int)(((uint32_t)b & vm) | ((uint32_t)last_b & ~vm));
+            }
+            // low bytes of both halves
+            *reinterpret_cast<uint16_t*>(obase + s * LR) =
+                (uint16_t)__builtin_amdgcn_perm(0u, pk_bits(o), 0x0c0c0200u);
+        }
+        __builtin_amdgcn_wave_barrier();
+        {   // this block's results, one window per lane
+            const int step = blk * K + lstep;
+            if (lline < nlines && step < n) {
+                uint4* dst = reinterpret_cast<uint4*>(out + pix_of(lline, step) * LR);
+                const uint4* src = reinterpret_cast<const uint4*>(S.out + lane * LR);
+#pragma unroll
+                for (int qv = 0; qv < NV; ++qv) dst[qv] = src[qv];
+            }
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < kPkAhead; ++u) fetch(u, u);
+    for (int blk0 = 0; blk0 < nblk; blk0 += kPkAhead) {
+#pragma unroll
+        for (int u = 0; u < kPkAhead; ++u) {
+            const int blk = blk0 + u;
+            if (blk < nblk) {
+                __builtin_amdgcn_wave_barrier();
+                stage(u);
+                if (blk + kPkAhead < nblk) fetch(u, blk + kPkAhead);
+                process(blk);
+            }
+        }
+    }
+}
+
+template <int LR>
+__global__ __launch_bounds__(64) void agg_init_pk_kernel(PyrLevel a) {
+    __shared__ __align__(16) AggInitPkLds<LR> lds;
+    constexpr int SPW = 128 / LR;
+    const int nh = (a.h + SPW - 1) / SPW, nv = (a.w + SPW - 1) / SPW;
+    const int item = blockIdx.x;
+    int dir, group;
+    if (item < nh) { dir = 1; group = item; }
+    else if (item < 2 * nh) { dir = 3; group = item - nh; }
+    else if (item < 2 * nh + nv) { dir = 0; group = item - 2 * nh; }
+    else { dir = 2; group = item - 2 * nh - nv; }
+    agg_init_pk_lines<LR>(a, dir, group, threadIdx.x, lds);
+}
+
+// The packed aggregation with four disparities per lane (two u16 pairs, A =
+// 4k, 4k+1 and B = 4k+2, 4k+3): LR / 4 lanes per scanline, 256 / LR
+// scanlines per wave.  Per step a lane needs the neighbour pairs starting
+// at 4k + q - 1, 4k + q + 1 and 4k + q + 3 (q = b - b'), the middle one
+// serving both of its pairs, so a step costs about as many instructions as
+// the two-disparity form for twice the cells.  It has half as many waves,
+// which suits the largest level; the launch picks the form.  Same window
+// row (overlapping pairs, 127 pads), table and exactness range.
+template <int LR>
+struct AggInitPk2Lds {
+    static constexpr int LPL = LR / 4, SPW = 64 / LPL, K = LR / 2;
+    static constexpr int PAD = (LR + 2 + 3) / 4 * 4;         // a multiple of 4: 16-byte row stores
+    static constexpr int ROW = (PAD + 2 * LR + 1 + 3) / 4 * 4;
+    uint32_t cost[SPW][LR / 2][K];           // [line][pair][step]: the pair's costs as u16
+    uint8_t out[2 * 64 * LR];                // [line][step][t]
+    int16_t init[2 * 64];                    // [line][step]
+    uint32_t pen2[kPkDeltaHi - kPkDeltaLo + 1];
+    uint32_t win[SPW][ROW];
+};
+
+template <int LR>
+__device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, int group, int lane,
+                                                   AggInitPk2Lds<LR>& S) {
+    using Lds = AggInitPk2Lds<LR>;
+    constexpr int LPL = Lds::LPL, SPW = Lds::SPW, K = Lds::K, PAD = Lds::PAD, ROW = Lds::ROW;
+    constexpr int NV = LR / 16;
+    constexpr uint32_t kPadPair = 127u * 0x10001u;
+    static_assert(SPW * K == 128, "a block is two pixels per lane");
+    const int sub = lane / LPL, k = lane % LPL;      // recurrence: line, disparity quad
+    const bool horiz = (dir == 1 || dir == 3);
+    const bool fwd = (dir == 0 || dir == 1);
+    const int W = a.w, H = a.h;
+    const int nlines = horiz ? H : W;
+    const int n = horiz ? W : H;
+    auto pix_of = [&](int line, int step) -> int64_t {
+        line = min(line, nlines - 1);
+        step = min(step, n - 1);
+        const int pos = fwd ? step : n - 1 - step;
+        return horiz ? (int64_t)line * W + pos : (int64_t)pos * W + line;
+    };
+    const int line = min(group * SPW + sub, nlines - 1);
+    uint8_t* out = (dir == 0) ? a.path[0] : (dir == 1) ? a.path[1] : (dir == 2) ? a.path[2] : a.path[3];
+
+    // loads and stores: lane j handles the block's pixels j and j + 64 in
+    // [line][step] order
+    // a ring of kPkAhead blocks of loads in flight (slot = block mod kPkAhead)
+    uint4 nc[kPkAhead][2][NV];
+    int nb[kPkAhead][2];
+    auto fetch = [&](int slot, int blk) __attribute__((always_inline)) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int idx = lane + 64 * h;
+            const int64_t p = pix_of(group * SPW + idx / K, blk * K + idx % K);
+            const uint4* src = reinterpret_cast<const uint4*>(a.cost + p * LR);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) nc[slot][h][v] = src[v];
+            nb[slot][h] = a.init[p];
+        }
+    };
+    auto stage = [&](int slot) __attribute__((always_inline)) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int idx = lane + 64 * h;
+            uint32_t wv[LR / 4];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                wv[4 * v] = nc[slot][h][v].x; wv[4 * v + 1] = nc[slot][h][v].y;
+                wv[4 * v + 2] = nc[slot][h][v].z; wv[4 * v + 3] = nc[slot][h][v].w;
+            }
+#pragma unroll
+            for (int j = 0; j < LR / 2; ++j)
+                S.cost[idx / K][j][idx % K] =
+                    __builtin_amdgcn_perm(0u, wv[j / 2], (j & 1) ? 0x0c030c02u : 0x0c010c00u);
+            S.init[idx] = (int16_t)nb[slot][h];
+        }
+    };
+
+    for (int e = lane; e < SPW * ROW; e += 64) (&S.win[0][0])[e] = kPadPair;
+    for (int e = lane; e <= kPkDeltaHi - kPkDeltaLo; e += 64) {
+        const int q = e + kPkDeltaLo;
+        const uint32_t v = (uint32_t)max(a.p1, a.p2 / max(1, abs(1 - q)));
+        S.pen2[e] = v | (v << 16);
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    uint32_t* wrow = &S.win[sub][0];
+    const uint32_t* wlane = wrow + (PAD - 1 + 4 * k);     // + q: the first neighbour pair
+    const bool first_k = k == 0, last_k = k == LPL - 1;
+    uint8_t* obase = S.out + sub * K * LR + 4 * k;
+    const pk16 p1v = pk_of((uint32_t)a.p1 * 0x10001u);
+    const pk16 m127 = pk_of(kPadPair);
+    const pk16 pen2_m1 = pk_of(S.pen2[-1 - kPkDeltaLo]);
+    const pk16 pen2_01 = pk_of(S.pen2[0 - kPkDeltaLo]);
+    int qlo = kPkDeltaLo, qhi = kPkDeltaHi, wlo = -(LR + 1);
+    asm volatile("" : "+v"(qlo), "+v"(qhi), "+v"(wlo));
+    // the window row from this lane's pairs of L + P1 (A, B): C[PAD + 4k ..
+    // PAD + 4k + 3] in one 16-byte store, and C[PAD - 1] from lane 0
+    auto put = [&](uint32_t oa, uint32_t ob) __attribute__((always_inline)) {
+        uint32_t nxt = dppu<0x101>(oa);                   // row_shl:1, lane k + 1's A
+        nxt = last_k ? kPadPair : nxt;
+        const uint4 v = make_uint4(oa, __builtin_amdgcn_alignbit(ob, oa, 16), ob,
+                                   __builtin_amdgcn_alignbit(nxt, ob, 16));
+        *reinterpret_cast<uint4*>(wrow + PAD + 4 * k) = v;
+        if (k == 0) wrow[PAD - 1] = (oa << 16) | 127u;
+    };
+    auto group_min4 = [&](pk16 v) __attribute__((always_inline)) {
+        v = __builtin_elementwise_min(v, v.yx);
+        uint32_t m = pk_bits(v);
+        m = min(m, dppu<0xB1>(m));                        // quad_perm [1, 0, 3, 2]
+        m = min(m, dppu<0x4E>(m));                        // quad_perm [2, 3, 0, 1]
+        if constexpr (LPL == 8) m = min(m, dppu<0x141>(m));   // row_half_mirror
+        return pk_of(m);
+    };
+
+    pk16 LA = pk_of(0u), LB = pk_of(0u), m = pk_of(0u);
+    int last_b = 0;
+    const int nblk = (n + K - 1) / K;
+    auto process = [&](int blk) __attribute__((always_inline)) {
+        __builtin_amdgcn_wave_barrier();
+        uint32_t bw[K / 2], ca[K], cb[K];
+#pragma unroll
+        for (int i = 0; i < K / 8; ++i) {
+            const uint4 v = reinterpret_cast<const uint4*>(&S.init[sub * K])[i];
+            bw[4 * i] = v.x; bw[4 * i + 1] = v.y; bw[4 * i + 2] = v.z; bw[4 * i + 3] = v.w;
+        }
+#pragma unroll
+        for (int i = 0; i < K / 4; ++i) {
+            const uint4 va = reinterpret_cast<const uint4*>(&S.cost[sub][2 * k][0])[i];
+            const uint4 vb = reinterpret_cast<const uint4*>(&S.cost[sub][2 * k + 1][0])[i];
+            ca[4 * i] = va.x; ca[4 * i + 1] = va.y; ca[4 * i + 2] = va.z; ca[4 * i + 3] = va.w;
+            cb[4 * i] = vb.x; cb[4 * i + 1] = vb.y; cb[4 * i + 2] = vb.z; cb[4 * i + 3] = vb.w;
+        }
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            const int b = (int)(int16_t)(bw[s / 2] >> (16 * (s & 1)));
+            const pk16 cA = pk_of(ca[s]), cB = pk_of(cb[s]);
+            pk16 oA, oB;
+            if (s == 0 && blk == 0) {
+                LA = cA;
+                LB = cB;
+                m = group_min4(__builtin_elementwise_min(LA, LB));
+                // A11: vertical directions seed b' from row 0 whichever end they start at
+                last_b = horiz ? b : a.init[line];
+                oA = LA;
+                oB = LB;
+            } else {
+                const bool valid = b != kInvalidShort;
+                const int q = b - last_b;
+                const uint32_t oa = pk_bits(LA + p1v), ob = pk_bits(LB + p1v);
+                uint32_t x0, x1, x2;                     // pairs at 4k + q - 1, + 1, + 3
+                pk16 pen2;
+                if (__builtin_amdgcn_ballot_w64(valid && (q < -1 || q > 1)) == 0) {
+                    uint32_t nxt = dppu<0x101>(oa);       // lane k + 1's A
+                    uint32_t prv = dppu<0x111>(ob);       // lane k - 1's B
+                    nxt = last_k ? kPadPair : nxt;
+                    prv = first_k ? kPadPair : prv;
+                    const uint32_t m1 = __builtin_amdgcn_alignbit(oa, prv, 16);    // 4k-1, 4k
+                    const uint32_t m3 = __builtin_amdgcn_alignbit(ob, oa, 16);     // 4k+1, 4k+2
+                    const uint32_t m5 = __builtin_amdgcn_alignbit(nxt, ob, 16);    // 4k+3, 4k+4
+                    const uint32_t mpos = 0u - (uint32_t)(q > 0), mzero = 0u - (uint32_t)(q == 0);
+                    x0 = (oa & mpos) | (((m1 & mzero) | (prv & ~mzero)) & ~mpos);
+                    x1 = (ob & mpos) | (((m3 & mzero) | (oa & ~mzero)) & ~mpos);
+                    x2 = (nxt & mpos) | (((m5 & mzero) | (ob & ~mzero)) & ~mpos);
+                    const uint32_t mneg = 0u - (uint32_t)(q < 0);
+                    pen2 = pk_of((pk_bits(pen2_m1) & mneg) | (pk_bits(pen2_01) & ~mneg));
+                } else {
+                    put(oa, ob);   // the window row holds L only when a step needs it
+                    const int qt = min(max(q, qlo), qhi);
+                    const int qw = min(max(q, wlo), LR + 1);
+                    x0 = wlane[qw];
+                    x1 = wlane[qw + 2];
+                    x2 = wlane[qw + 4];
+                    uint32_t pb = S.pen2[qt - kPkDeltaLo];
+                    // complete these reads inside the branch (see agg_init_pk_lines)
+                    asm volatile("" : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(pb));
+                    pen2 = pk_of(pb);
+                }
+                const pk16 t = __builtin_elementwise_min(pk_of(x1), m + pen2);
+                const pk16 vA = __builtin_elementwise_min(__builtin_elementwise_min(LA, pk_of(x0)), t);
+                const pk16 vB = __builtin_elementwise_min(__builtin_elementwise_min(LB, pk_of(x2)), t);
+                const pk16 lA = cA + (vA - m), lB = cB + (vB - m);
+                const pk16 mn = group_min4(__builtin_elementwise_min(lA, lB));
+                const uint32_t vm = 0u - (uint32_t)valid;
+                oA = pk_of(pk_bits(lA) & vm);
+                oB = pk_of(pk_bits(lB) & vm);
+                LA = pk_of((pk_bits(lA) & vm) | (pk_bits(LA) & ~vm));
+                LB = pk_of((pk_bits(lB) & vm) | (pk_bits(LB) & ~vm));
+                m = pk_of((pk_bits(mn) & vm) | (pk_bits(m127) & ~vm));
+                last_b = (int)(((uint32_t)b & vm) | ((uint32_t)last_b & ~vm));
+            }
+            // low bytes of the four halves
+            *reinterpret_cast<uint32_t*>(obase + s * LR) =
+                __builtin_amdgcn_perm(pk_bits(oB), pk_bits(oA), 0x06040200u);
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {   // this block's results, one window per pixel
+            const int idx = lane + 64 * h;
+            const int ll = group * SPW + idx / K, step = blk * K + idx % K;
+            if (ll < nlines && step < n) {
+                uint4* dst = reinterpret_cast<uint4*>(out + pix_of(ll, step) * LR);
+                const uint4* src = reinterpret_cast<const uint4*>(S.out + idx * LR);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) dst[v] = src[v];
+            }
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < kPkAhead; ++u) fetch(u, u);
+    for (int blk0 = 0; blk0 < nblk; blk0 += kPkAhead) {
+#pragma unroll
+        for (int u = 0; u < kPkAhead; ++u) {
+            const int blk = blk0 + u;
+            if (blk < nblk) {
+                __builtin_amdgcn_wave_barrier();
+                stage(u);
+                if (blk + kPkAhead < nblk) fetch(u, blk + kPkAhead);
+                process(blk);
+            }
+        }
+    }
+}
+
+template <int LR>
+__global__ __launch_bounds__(64) void agg_init_pk2_kernel(PyrLevel a) {
+    __shared__ __align__(16) AggInitPk2Lds<LR> lds;
+    constexpr int SPW = AggInitPk2Lds<LR>::SPW;
+    const int nh = (a.h + SPW - 1) / SPW, nv = (a.w + SPW - 1) / SPW;
+    const int item = blockIdx.x;
+    int dir, group;
+    if (item < nh) { dir = 1; group = item; }
+    else if (item < 2 * nh) { dir = 3; group = item - nh; }
+    else if (item < 2 * nh + nv) { dir = 0; group = item - 2 * nh; }
+    else { dir = 2; group = item - 2 * nh - nv; }
+    agg_init_pk2_lines<LR>(a, dir, group, threadIdx.x, lds);
+}
+
 // WTA with initial disparity (cusgm_ca.cu:3425-3512), one lane per pixel
 template <int LR>
 __global__ __launch_bounds__(256) void wta_init_kernel(PyrLevel a) {
@@ -352,13 +852,54 @@ hipError_t pyr_launch_cost_init(const PyrLevel& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Whether the packed-u16 window aggregation is exact for these penalties.
+static bool agg_init_packed_ok(const PyrLevel& a) { return a.p1 <= 127 && a.p2 <= 254; }
+
+static std::atomic<int> g_window_form{SGM_WINDOW_AUTO};
+int window_form() { return g_window_form.load(); }
+void set_window_form(int form) { g_window_form.store(form); }
+
+// SIMDs of the current device (4 per compute unit), looked up once per device
+static int device_simds() {
+    static std::mutex mu;
+    static int simds[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!simds[dev]) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        simds[dev] = 4 * cus;
+    }
+    return simds[dev];
+}
+
 hipError_t pyr_launch_aggregate_init(const PyrLevel& a, hipStream_t s) {
+    auto items = [&](int spw) { return 2 * ((a.h + spw - 1) / spw) + 2 * ((a.w + spw - 1) / spw); };
+    int form = window_form();
+    if (!agg_init_packed_ok(a)) form = SGM_WINDOW_U32;
+    if (form == SGM_WINDOW_AUTO)
+        form = items(256 / a.level_range) >= device_simds() ? SGM_WINDOW_QUADS : SGM_WINDOW_PAIRS;
+    if (form == SGM_WINDOW_QUADS) {
+        const int n = items(256 / a.level_range);
+        if (a.level_range == 16) hipLaunchKernelGGL(agg_init_pk2_kernel<16>, dim3(n), dim3(64), 0, s, a);
+        else hipLaunchKernelGGL(agg_init_pk2_kernel<32>, dim3(n), dim3(64), 0, s, a);
+        return hipGetLastError();
+    }
+    if (form == SGM_WINDOW_PAIRS) {
+        const int n = items(128 / a.level_range);
+        if (a.level_range == 16) hipLaunchKernelGGL(agg_init_pk_kernel<16>, dim3(n), dim3(64), 0, s, a);
+        else hipLaunchKernelGGL(agg_init_pk_kernel<32>, dim3(n), dim3(64), 0, s, a);
+        return hipGetLastError();
+    }
     if (a.level_range == 16) {
-        const int items = 2 * ((a.h + 3) / 4) + 2 * ((a.w + 3) / 4);
-        hipLaunchKernelGGL(agg_init_kernel<16>, dim3(items), dim3(64), 0, s, a);
+        const int n = items(4);
+        hipLaunchKernelGGL(agg_init_kernel<16>, dim3(n), dim3(64), 0, s, a);
     } else {
-        const int items = 2 * ((a.h + 1) / 2) + 2 * ((a.w + 1) / 2);
-        hipLaunchKernelGGL(agg_init_kernel<32>, dim3(items), dim3(64), 0, s, a);
+        const int n = items(2);
+        hipLaunchKernelGGL(agg_init_kernel<32>, dim3(n), dim3(64), 0, s, a);
     }
     return hipGetLastError();
 }

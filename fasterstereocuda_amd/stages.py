@@ -134,6 +134,20 @@ def cost_init(census_l, census_r, init, level_range: int, cs_type: int = 1) -> t
     return out
 
 
+WINDOW_AUTO, WINDOW_U32, WINDOW_PAIRS, WINDOW_QUADS = 0, 1, 2, 3
+
+
+def set_window_form(form: int) -> None:
+    """How the window aggregation holds a scanline's disparities (one per
+    lane in 32-bit arithmetic, two or four per lane as packed u16, or AUTO);
+    process-wide, read at every launch.  Results are byte-identical."""
+    check(lib().sgm_set_window_form(int(form)))
+
+
+def get_window_form() -> int:
+    return int(lib().sgm_get_window_form())
+
+
 def aggregate_init(cost_vol, init, p1: int, p2: int):
     _dev(cost_vol, torch.uint8)
     _dev(init, torch.int16)

@@ -164,6 +164,24 @@ void sgm_roi_from_disparity(const float* disp, int32_t width, int32_t height, sg
 int sgm_set_cost_mode(int32_t mode);
 int32_t sgm_get_cost_mode(void);
 
+/* How the window aggregation of the pyramid's finer levels (and
+ * sgm_stage_aggregate_init) holds the disparities of a scanline (no
+ * reference counterpart; cusgm_ca.cu:1181-1339 is the arithmetic):
+ *   SGM_WINDOW_U32    one per lane in 32-bit arithmetic (any P1, P2)
+ *   SGM_WINDOW_PAIRS  two per lane as packed u16 (P1 <= 127, P2 <= 254)
+ *   SGM_WINDOW_QUADS  four per lane as two packed u16 pairs (same range)
+ *   SGM_WINDOW_AUTO   the packed form with the fewest instructions that
+ *                     still gives the device a wave per SIMD, the 32-bit
+ *                     form outside the packed range
+ * A packed form asked for outside its range runs the 32-bit form.
+ * Process-wide, read at every launch.  All give byte-identical results. */
+#define SGM_WINDOW_AUTO 0
+#define SGM_WINDOW_U32 1
+#define SGM_WINDOW_PAIRS 2
+#define SGM_WINDOW_QUADS 3
+int sgm_set_window_form(int32_t form);
+int32_t sgm_get_window_form(void);
+
 /* Row strips of the peak removal (do_remove_peaks; StereoCudaImpl.cpp:32-97,
  * 463-498).  The reference splits the left map into
  * hardware_concurrency() * 3 / 4 strips of H / strips rows (rows past the last

@@ -197,6 +197,16 @@ def test_peak_strips_setting():
     assert lib.sgm_set_peak_strips(1) == 0
 
 
+def test_window_form_setting():
+    lib = _lib.lib()
+    assert lib.sgm_get_window_form() == 0          # SGM_WINDOW_AUTO
+    for form in (1, 2, 3, 0):
+        assert lib.sgm_set_window_form(form) == 0 and lib.sgm_get_window_form() == form
+    assert lib.sgm_set_window_form(4) == _lib.SGM_E_INVALID_ARG
+    assert lib.sgm_set_window_form(-1) == _lib.SGM_E_INVALID_ARG
+    assert lib.sgm_get_window_form() == 0
+
+
 def test_uninitialised_handle_and_constants():
     st = StereoCuda()
     import numpy as np

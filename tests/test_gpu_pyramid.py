@@ -17,6 +17,18 @@ def _bits(a):
     return np.asarray(a, np.float32).view(np.uint32)
 
 
+@pytest.fixture(params=[stages.WINDOW_AUTO, stages.WINDOW_U32, stages.WINDOW_PAIRS, stages.WINDOW_QUADS],
+                ids=["auto", "u32", "pairs", "quads"])
+def window_form(request):
+    """Runs a test under each form of the window aggregation (one disparity
+    per lane in u32, two or four per lane as packed u16, the automatic
+    choice); all must give the same bytes."""
+    before = stages.get_window_form()
+    stages.set_window_form(request.param)
+    yield request.param
+    stages.set_window_form(before)
+
+
 def _random_init(rng, h, w, lr, invalid_frac=0.15):
     init = rng.integers(-3, 50, size=(h, w)).astype(np.int16)
     # piecewise-constant runs with jumps, like propagated disparities
@@ -27,8 +39,8 @@ def _random_init(rng, h, w, lr, invalid_frac=0.15):
 
 @pytest.mark.parametrize("lr", [16, 32])
 @pytest.mark.parametrize("cs", [1, 0])
-def test_window_stages_match_oracle(cuda, lr, cs):
-    print(f"[window stages] lr={lr} cs={cs}", flush=True)
+def test_window_stages_match_oracle(cuda, window_form, lr, cs):
+    print(f"[window stages] lr={lr} cs={cs} form={window_form}", flush=True)
     rng = np.random.default_rng(lr + cs)
     pair = scenes.cone_like(2, H=57, W=131, D=64)
     h, w = pair.left.shape
@@ -52,10 +64,10 @@ def test_window_stages_match_oracle(cuda, lr, cs):
 
 @pytest.mark.parametrize("lr", [16, 32])
 @pytest.mark.parametrize("p1,p2", [(10, 254), (20, 400), (0, 60000)])
-def test_window_aggregation_penalty_forms(cuda, lr, p1, p2):
+def test_window_aggregation_penalty_forms(cuda, window_form, lr, p1, p2):
     """P2 / max(1, |b' - b + 1|) comes from a table up to P2 = 254 and from a
     division above; both against the oracle, with base jumps of every size."""
-    print(f"[window aggregation] lr={lr} p1={p1} p2={p2}", flush=True)
+    print(f"[window aggregation] lr={lr} p1={p1} p2={p2} form={window_form}", flush=True)
     rng = np.random.default_rng(lr * 7 + p2)
     h, w = 41, 203
     init = rng.integers(-40, 300, size=(h, w)).astype(np.int16)   # large jumps
@@ -67,6 +79,63 @@ def test_window_aggregation_penalty_forms(cuda, lr, p1, p2):
     for r in range(4):
         np.testing.assert_array_equal(paths[r].cpu().numpy(),
                                       oracle.aggregate_init(cost, init, r, p1, p2), err_msg=f"dir {r}")
+
+
+@pytest.mark.parametrize("lr", [16, 32])
+@pytest.mark.parametrize("p1,p2", [(127, 254), (127, 127), (0, 0), (1, 254), (128, 254), (127, 255)])
+def test_window_aggregation_packed_limits(cuda, window_form, lr, p1, p2):
+    """The packed-u16 window aggregation (P1 <= 127, P2 <= 254) at the edges
+    of that range, and the first penalties past it (the u32 kernel): costs up
+    to 255 with many at 255, INVALID bases in runs and at the first pixel of
+    scanlines, base jumps far beyond the window, each direction bit for bit
+    against the oracle."""
+    print(f"[window aggregation limits] lr={lr} p1={p1} p2={p2} form={window_form}", flush=True)
+    rng = np.random.default_rng(1000 + lr + p1 * 3 + p2)
+    h, w = 37, 149
+    init = rng.integers(-300, 300, size=(h, w)).astype(np.int16)
+    init = np.where(rng.random((h, w)) < 0.5, np.roll(init, 1, axis=1), init).astype(np.int16)
+    init[rng.random((h, w)) < 0.12] = oracle.INVALID_SHORT
+    init[:, 0] = oracle.INVALID_SHORT            # horizontal scanlines start invalid
+    init[0, ::2] = oracle.INVALID_SHORT          # vertical ones too, every other column
+    init[5:9, 20:60] = oracle.INVALID_SHORT      # a block of invalid bases
+    cost = rng.integers(0, 256, size=(h, w, lr), dtype=np.uint8)
+    cost[rng.random((h, w, lr)) < 0.3] = 255
+    paths = stages.aggregate_init(torch.from_numpy(cost).to(cuda), torch.from_numpy(init).to(cuda),
+                                  p1, p2)
+    for r in range(4):
+        np.testing.assert_array_equal(paths[r].cpu().numpy(),
+                                      oracle.aggregate_init(cost, init, r, p1, p2), err_msg=f"dir {r}")
+
+
+@pytest.mark.parametrize("lr", [16, 32])
+@pytest.mark.parametrize("jump_rate", [0.0, 0.02])
+def test_window_aggregation_small_base_steps(cuda, window_form, lr, jump_rate):
+    """Bases that move by -1, 0 or +1 from pixel to pixel along both axes (a
+    random walk, as propagated disparities of smooth surfaces do), with rare
+    large jumps and INVALID runs: steps where every scanline of a wave
+    shifts by at most one disparity and steps where one does not, mixed
+    within the same scanlines.  Each direction bit for bit against the
+    oracle."""
+    print(f"[window aggregation small steps] lr={lr} jump_rate={jump_rate} form={window_form}", flush=True)
+    rng = np.random.default_rng(77 + lr + int(jump_rate * 100))
+    h, w = 53, 211
+    walk_x = np.cumsum(rng.integers(-1, 2, size=(1, w)), axis=1)
+    walk_y = np.cumsum(rng.integers(-1, 2, size=(h, 1)), axis=0)
+    init = (40 + walk_x + walk_y).astype(np.int64)
+    init += (rng.random((h, w)) < 0.05).astype(np.int64)        # isolated +1 bumps
+    jumps = rng.random((h, w)) < jump_rate
+    init[jumps] += rng.integers(-60, 60, size=int(jumps.sum()))
+    init = init.astype(np.int16)
+    init[rng.random((h, w)) < 0.03] = oracle.INVALID_SHORT
+    init[10:12, 50:70] = oracle.INVALID_SHORT
+    cost = rng.integers(0, 256, size=(h, w, lr), dtype=np.uint8)
+    for p1, p2 in ((10, 150), (127, 254)):
+        paths = stages.aggregate_init(torch.from_numpy(cost).to(cuda), torch.from_numpy(init).to(cuda),
+                                      p1, p2)
+        for r in range(4):
+            np.testing.assert_array_equal(paths[r].cpu().numpy(),
+                                          oracle.aggregate_init(cost, init, r, p1, p2),
+                                          err_msg=f"p1={p1} p2={p2} dir {r}")
 
 
 @pytest.mark.parametrize("w,h", [(130, 57), (131, 58), (64, 33)])
