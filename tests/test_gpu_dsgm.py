@@ -28,9 +28,11 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-# shapes cover segment boundaries: lines of 32, 33, 40, 64 and 97 steps
+# even and odd widths and heights, lines of fewer steps than the load
+# pipeline's depth (rows of 1, 2 and 3 pixels), and D = 64, 128, 256
 @pytest.mark.parametrize("B,H,W,D", [(2, 33, 40, 64), (1, 32, 97, 128), (1, 20, 64, 256),
-                                     (2, 45, 70, 128)])
+                                     (2, 45, 70, 128), (1, 7, 1, 64), (2, 5, 2, 64),
+                                     (1, 9, 3, 128)])
 def test_forward_backward_match_fp64(cuda, B, H, W, D):
     print(f"[dsgm] B{B} H{H} W{W} D{D}", flush=True)
     cost, p1, p2, gt = _inputs(B, H, W, D, seed=H + W + D)
