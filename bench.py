@@ -33,6 +33,12 @@ sys.path.insert(0, ROOT)
 # the CPU baseline's OpenMP threads stay on neighbouring cores; set before any
 # OpenMP runtime starts (the thread count is the environment's own)
 os.environ.setdefault("OMP_PROC_BIND", "close")
+# CPUs this process may run on, read before an OpenMP runtime binds the main
+# thread to a single place (afterwards the main thread's mask is one CPU)
+try:
+    CPUS_USABLE = len(os.sched_getaffinity(0))
+except (AttributeError, OSError):
+    CPUS_USABLE = None
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per cell and view of aggregation + WTA: 4 direction reads
@@ -268,10 +274,6 @@ def cpu_baseline(pairs_np, D, gpu_first, runs, per_run):
             oracle.match(p.left, p.right, 0, D)
         rates.append(per_run * W * H * D / (time.perf_counter() - t0))
     phys, logical = host_cores()
-    try:
-        affinity = len(os.sched_getaffinity(0))
-    except (AttributeError, OSError):
-        affinity = None
     return {
         "value": statistics.median(rates),
         "unit": "W*H*D/s",
@@ -285,7 +287,7 @@ def cpu_baseline(pairs_np, D, gpu_first, runs, per_run):
         "cpu_model": cpu_model(),
         "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
         "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
-        "cpus_usable": affinity,
+        "cpus_usable": CPUS_USABLE,
         "machine_physical_cores": phys,
         "machine_logical_cpus": logical,
     }
