@@ -229,7 +229,8 @@ enum FwdMode {
 // block).  Steps past the end load the last pixel again and are never used.
 constexpr int kAheadF = 4;
 
-// One direction over a scanline group (forward), writing its checkpoints.
+// One direction over a scanline group (forward), saving each step's winner
+// codes and the previous step's argmin for the backward.
 template <int D, int MODE>
 __device__ __forceinline__ void fwd_lines(const DsgmArgs& A, int b, int path, int group, int lane) {
     constexpr int DPD = D / 16;
