@@ -27,9 +27,11 @@ __device__ __forceinline__ uint32_t bperm(int src_lane, uint32_t v) {
     return (uint32_t)__builtin_amdgcn_ds_bpermute(src_lane << 2, (int)v);
 }
 
+// lanes whose source is outside the row read 0 (bound_ctrl), so the move
+// needs no zeroed destination first
 template <int CTRL>
 __device__ __forceinline__ uint32_t dppu(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
 }
 
 // minimum over each aligned group of LR lanes (LR = 16 or 32)
@@ -301,6 +303,23 @@ struct AggInitPkLds {
 __device__ __forceinline__ uint32_t pk_bits(pk16 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ pk16 pk_of(uint32_t v) { return __builtin_bit_cast(pk16, v); }
 
+// v_perm selector of bytes 2 (q + 1) .. 2 (q + 1) + 3 of {hi, lo} for a
+// window shift q in {-1, 0, 1}: the dword starting one disparity below, at,
+// or one above the middle of the pair boundary.  Other q (lanes whose step
+// is skipped) are clamped into that range; their result is not used.
+// Three full-rate instructions: q clamped by v_med3, then 24-bit multiply-adds
+// (written out: the compiler merges the steps into one quarter-rate 32-bit
+// multiply).  The constants come in registers (no literals in VOP3).
+__device__ __forceinline__ uint32_t shift_sel(int q, uint32_t k0202, uint32_t k10001, uint32_t kbase) {
+    uint32_t t, u, sel;
+    asm("v_med3_i32 %0, %3, -1, 1\n\t"
+        "v_mad_i32_i24 %1, %0, %4, %4\n\t"     // (q + 1) * 0x0202
+        "v_mad_u32_u24 %2, %1, %5, %6"           // u * 0x10001 + 0x03020100
+        : "=&v"(t), "=&v"(u), "=v"(sel)
+        : "v"(q), "s"(k0202), "v"(k10001), "s"(kbase));
+    return sel;
+}
+
 // minimum over each aligned group of LR / 2 lanes of both halves; the result
 // holds it in both halves
 template <int LR>
@@ -379,6 +398,7 @@ __device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, in
     uint8_t* obase = S.out + sub * K * LR + 2 * k;
     const pk16 p1v = pk_of((uint32_t)a.p1 * 0x10001u);
     const pk16 m127 = pk_of(kPadPair);
+    const uint32_t k10001 = 0x10001u;
     // clamp bounds kept in registers (one v_med3 each per step) instead of
     // rematerialised each step
     int qlo = kPkDeltaLo, qhi = kPkDeltaHi, wlo = -(LR + 1);
@@ -433,20 +453,18 @@ __device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, in
                 const int q = b - last_b;
                 const uint32_t own = pk_bits(L + p1v);
                 pk16 bm, bp, pen2;                        // neighbour terms, + P1 already
-                if (__builtin_amdgcn_ballot_w64(valid && (q < -1 || q > 1)) == 0) {
+                if (__builtin_expect(__builtin_amdgcn_ballot_w64(valid && (q < -1 || q > 1)) == 0, 1)) {
                     // every line shifts by at most one disparity: the terms are
                     // this lane's pair and its neighbours' (DPP), no LDS
                     uint32_t nxt = dppu<0x101>(own);          // row_shl:1, lane k + 1
                     uint32_t prv = dppu<0x111>(own);          // row_shr:1, lane k - 1
                     nxt = last_k ? kPadPair : nxt;
                     prv = first_k ? kPadPair : prv;
-                    const uint32_t lo_mid = __builtin_amdgcn_alignbit(own, prv, 16);   // 2k-1, 2k
-                    const uint32_t hi_mid = __builtin_amdgcn_alignbit(nxt, own, 16);   // 2k+1, 2k+2
-                    const uint32_t mpos = 0u - (uint32_t)(q > 0), mzero = 0u - (uint32_t)(q == 0);
-                    const uint32_t bm0 = (lo_mid & mzero) | (prv & ~mzero);   // q = 0 or -1
-                    const uint32_t bp0 = (hi_mid & mzero) | (own & ~mzero);
-                    bm = pk_of((own & mpos) | (bm0 & ~mpos));
-                    bp = pk_of((nxt & mpos) | (bp0 & ~mpos));
+                    // the pair starting at disparity 2k + q - 1 (2k + q + 1) is
+                    // bytes 2 (q + 1) .. 2 (q + 1) + 3 of {own, prv} ({nxt, own})
+                    const uint32_t sel = shift_sel(q, 0x0202u, k10001, 0x03020100u);
+                    bm = pk_of(__builtin_amdgcn_perm(own, prv, sel));
+                    bp = pk_of(__builtin_amdgcn_perm(nxt, own, sel));
                     const uint32_t mneg = 0u - (uint32_t)(q < 0);
                     pen2 = pk_of((pk_bits(pen2_m1) & mneg) | (pk_bits(pen2_01) & ~mneg));
                 } else {
@@ -610,6 +628,7 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
     uint8_t* obase = S.out + sub * K * LR + 4 * k;
     const pk16 p1v = pk_of((uint32_t)a.p1 * 0x10001u);
     const pk16 m127 = pk_of(kPadPair);
+    const uint32_t k10001 = 0x10001u;
     const pk16 pen2_m1 = pk_of(S.pen2[-1 - kPkDeltaLo]);
     const pk16 pen2_01 = pk_of(S.pen2[0 - kPkDeltaLo]);
     int qlo = kPkDeltaLo, qhi = kPkDeltaHi, wlo = -(LR + 1);
@@ -655,33 +674,31 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
         for (int s = 0; s < K; ++s) {
             const int b = (int)(int16_t)(bw[s / 2] >> (16 * (s & 1)));
             const pk16 cA = pk_of(ca[s]), cB = pk_of(cb[s]);
-            pk16 oA, oB;
+            uint32_t ow;   // the low bytes of the four halves
             if (s == 0 && blk == 0) {
                 LA = cA;
                 LB = cB;
                 m = group_min4(__builtin_elementwise_min(LA, LB));
                 // A11: vertical directions seed b' from row 0 whichever end they start at
                 last_b = horiz ? b : a.init[line];
-                oA = LA;
-                oB = LB;
+                ow = __builtin_amdgcn_perm(pk_bits(LB), pk_bits(LA), 0x06040200u);
             } else {
                 const bool valid = b != kInvalidShort;
                 const int q = b - last_b;
                 const uint32_t oa = pk_bits(LA + p1v), ob = pk_bits(LB + p1v);
                 uint32_t x0, x1, x2;                     // pairs at 4k + q - 1, + 1, + 3
                 pk16 pen2;
-                if (__builtin_amdgcn_ballot_w64(valid && (q < -1 || q > 1)) == 0) {
+                if (__builtin_expect(__builtin_amdgcn_ballot_w64(valid && (q < -1 || q > 1)) == 0, 1)) {
                     uint32_t nxt = dppu<0x101>(oa);       // lane k + 1's A
                     uint32_t prv = dppu<0x111>(ob);       // lane k - 1's B
                     nxt = last_k ? kPadPair : nxt;
                     prv = first_k ? kPadPair : prv;
-                    const uint32_t m1 = __builtin_amdgcn_alignbit(oa, prv, 16);    // 4k-1, 4k
-                    const uint32_t m3 = __builtin_amdgcn_alignbit(ob, oa, 16);     // 4k+1, 4k+2
-                    const uint32_t m5 = __builtin_amdgcn_alignbit(nxt, ob, 16);    // 4k+3, 4k+4
-                    const uint32_t mpos = 0u - (uint32_t)(q > 0), mzero = 0u - (uint32_t)(q == 0);
-                    x0 = (oa & mpos) | (((m1 & mzero) | (prv & ~mzero)) & ~mpos);
-                    x1 = (ob & mpos) | (((m3 & mzero) | (oa & ~mzero)) & ~mpos);
-                    x2 = (nxt & mpos) | (((m5 & mzero) | (ob & ~mzero)) & ~mpos);
+                    // pairs at 4k + q - 1, 4k + q + 1, 4k + q + 3: bytes
+                    // 2 (q + 1) .. 2 (q + 1) + 3 of {oa, prv}, {ob, oa}, {nxt, ob}
+                    const uint32_t sel = shift_sel(q, 0x0202u, k10001, 0x03020100u);
+                    x0 = __builtin_amdgcn_perm(oa, prv, sel);
+                    x1 = __builtin_amdgcn_perm(ob, oa, sel);
+                    x2 = __builtin_amdgcn_perm(nxt, ob, sel);
                     const uint32_t mneg = 0u - (uint32_t)(q < 0);
                     pen2 = pk_of((pk_bits(pen2_m1) & mneg) | (pk_bits(pen2_01) & ~mneg));
                 } else {
@@ -702,16 +719,13 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
                 const pk16 lA = cA + (vA - m), lB = cB + (vB - m);
                 const pk16 mn = group_min4(__builtin_elementwise_min(lA, lB));
                 const uint32_t vm = 0u - (uint32_t)valid;
-                oA = pk_of(pk_bits(lA) & vm);
-                oB = pk_of(pk_bits(lB) & vm);
+                ow = __builtin_amdgcn_perm(pk_bits(lB), pk_bits(lA), 0x06040200u) & vm;
                 LA = pk_of((pk_bits(lA) & vm) | (pk_bits(LA) & ~vm));
                 LB = pk_of((pk_bits(lB) & vm) | (pk_bits(LB) & ~vm));
                 m = pk_of((pk_bits(mn) & vm) | (pk_bits(m127) & ~vm));
                 last_b = (int)(((uint32_t)b & vm) | ((uint32_t)last_b & ~vm));
             }
-            // low bytes of the four halves
-            *reinterpret_cast<uint32_t*>(obase + s * LR) =
-                __builtin_amdgcn_perm(pk_bits(oB), pk_bits(oA), 0x06040200u);
+            *reinterpret_cast<uint32_t*>(obase + s * LR) = ow;
         }
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
