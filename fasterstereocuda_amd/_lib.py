@@ -116,6 +116,8 @@ SIGNATURES = {
     "sgm_get_cost_mode": (I32, []),
     "sgm_set_window_form": (ctypes.c_int, [I32]),
     "sgm_get_window_form": (I32, []),
+    "sgm_set_wta_form": (ctypes.c_int, [I32]),
+    "sgm_get_wta_form": (I32, []),
     "sgm_batch_destroy": (None, [P]),
     "sgm_dsgm_checkpoint_floats": (ctypes.c_size_t, [I32, I32, I32, I32]),
     "sgm_dsgm_workspace_floats": (ctypes.c_size_t, [I32, I32, I32, I32]),

@@ -48,6 +48,7 @@ int set_global_error(int code, const std::string& msg) { return fail(nullptr, co
 
 namespace {
 std::atomic<int> g_cost_mode{0};
+std::atomic<int> g_wta_form{0};
 std::atomic<int> g_wide_min_groups{-1};
 std::atomic<int> g_peak_strips{1};
 }  // namespace
@@ -69,6 +70,27 @@ int wide_min_groups() {
         g_wide_min_groups.store(v);
     }
     return v;
+}
+
+// Stored-cost plans: the four directions and then the WTA, or bottom->top
+// fused with the WTA after the other three.  The fused bottom->top walk is
+// latency-bound on a small plan (about 250 ns per step), so when the
+// four-direction launch leaves the device at 2.5 waves per SIMD or fewer the
+// walk is better run beside the horizontal ones and the WTA left to its own
+// memory-bound launch -- unless the rows are much longer than the columns:
+// then the fused walk is short and the extra launch costs more than it saves
+// (measured on single pairs from 450x375 to 1920x1080).
+bool separate_wta(const StagePlan& p) {
+    const int f = g_wta_form.load();
+    if (f == SGM_WTA_FUSED) return false;
+    if (f == SGM_WTA_SEPARATE) return true;
+    return p.squarish && 4 * p.n_agg <= 5 * wide_min_groups();
+}
+
+bool squarish_jobs(const std::vector<SgmJob>& jobs, int n) {
+    for (int i = 0; i < n; ++i)
+        if (2 * (int64_t)(jobs[i].W - jobs[i].roi_x) > 5 * (int64_t)jobs[i].H) return false;
+    return n > 0;
 }
 
 bool choose_wide(bool eligible, int n_wh) {
@@ -187,6 +209,7 @@ int fsgm::upload_plan(const std::vector<SgmJob>& jobs, const std::vector<SgmPost
     plan->n_wv = wd[nj];
     plan->n_wf = wf[nj];
     plan->wide = choose_wide(wide, w3[nj]);
+    plan->squarish = squarish_jobs(jobs, nj);
     plan->n_post = op[np];
     plan->dmax = dmax;
     return SGM_OK;
@@ -212,11 +235,12 @@ void fsgm::restrict_matching(StagePlan* P, const std::vector<SgmJob>& jobs, int 
         P->n_dw += dw_items(jobs[i]);
     }
     P->wide = choose_wide(wide, P->n_wh);
+    P->squarish = squarish_jobs(jobs, n);
 }
 
-// census, cost, three directions, bottom->top + WTA, post, [right-map LR
-// invalidation]; marks[k] (may be null) are recorded before stage k and
-// after the last one.
+// census, cost, three directions, bottom->top + WTA (or the four directions,
+// then the WTA: separate_wta), post, [right-map LR invalidation]; marks[k]
+// (may be null) are recorded before stage k and after the last one.
 hipError_t fsgm::launch_match(const StagePlan& plan, bool right_invalidation, hipStream_t s,
                               hipEvent_t* marks, const AggSchedule* sched) {
     StagePlan ctw = plan;
@@ -229,14 +253,19 @@ hipError_t fsgm::launch_match(const StagePlan& plan, bool right_invalidation, hi
     // with the on-the-fly cost there is no cost volume to build
     if (!plan.wide && (e = launch_cost(ctw, s)) != hipSuccess) return e;
     if ((e = mark(2)) != hipSuccess) return e;
+    const bool separate = !plan.wide && !sched && separate_wta(ctw);
     if (plan.wide)
         e = sched ? launch_wide3_sched(ctw, *sched, s) : launch_wide3(ctw, s);
+    else if (separate)
+        e = launch_aggregate(ctw, s);
     else
         e = sched ? launch_aggregate_sched(ctw, *sched, s) : launch_aggregate3(ctw, s);
     if (e != hipSuccess) return e;
     if ((e = mark(3)) != hipSuccess) return e;
     if (plan.wide)
         e = sched ? launch_wide_wta_sched(ctw, *sched, s) : launch_wide_wta(ctw, s);
+    else if (separate)
+        e = launch_wta(ctw, s);
     else
         e = sched ? launch_aggregate_wta_sched(ctw, *sched, s) : launch_aggregate_wta(ctw, s);
     if (e != hipSuccess) return e;
@@ -675,6 +704,16 @@ int sgm_set_cost_mode(int32_t mode) {
 }
 
 int32_t sgm_get_cost_mode(void) { return fsgm::cost_mode(); }
+
+int sgm_set_wta_form(int32_t form) {
+    if (form != SGM_WTA_AUTO && form != SGM_WTA_FUSED && form != SGM_WTA_SEPARATE)
+        return fail(nullptr, SGM_E_INVALID_ARG,
+                    "WTA form must be SGM_WTA_AUTO, SGM_WTA_FUSED or SGM_WTA_SEPARATE");
+    fsgm::g_wta_form.store(form);
+    return SGM_OK;
+}
+
+int32_t sgm_get_wta_form(void) { return fsgm::g_wta_form.load(); }
 
 int sgm_set_peak_strips(int32_t strips) {
     if (strips < 1 || strips > 4096)

@@ -131,6 +131,7 @@ struct StagePlan {
     int n_wh = 0, n_wv = 0, n_wf = 0;
     int dmax = 64;
     bool wide = false;            // matching uses the on-the-fly cost kernels
+    bool squarish = false;        // every matched job has W - roi_x <= 2.5 H
     IoBind io{};                  // host-side: bound caller buffers, if any
 };
 

@@ -148,6 +148,21 @@ def get_window_form() -> int:
     return int(lib().sgm_get_window_form())
 
 
+WTA_AUTO, WTA_FUSED, WTA_SEPARATE = 0, 1, 2
+
+
+def set_wta_form(form: int) -> None:
+    """How the stored-cost aggregation reaches the WTA: bottom->top fused
+    with it after the other three directions, or the four directions in one
+    launch and then the WTA, or AUTO (separate for single pairs); process-wide,
+    read when the launches are enqueued.  Results are byte-identical."""
+    check(lib().sgm_set_wta_form(int(form)))
+
+
+def get_wta_form() -> int:
+    return int(lib().sgm_get_wta_form())
+
+
 def aggregate_init(cost_vol, init, p1: int, p2: int):
     _dev(cost_vol, torch.uint8)
     _dev(init, torch.int16)

@@ -182,6 +182,27 @@ int32_t sgm_get_cost_mode(void);
 int sgm_set_window_form(int32_t form);
 int32_t sgm_get_window_form(void);
 
+/* How the stored-cost aggregation reaches the WTA (no reference counterpart;
+ * the reference runs the four directions, then the WTA, cusgm_ca.cu:4080-4135):
+ *   SGM_WTA_FUSED     left->right, right->left and top->bottom in one launch,
+ *                     then bottom->top fused with the WTA: no bottom->top
+ *                     volume is written or read
+ *   SGM_WTA_SEPARATE  the four directions in one launch, then the WTA: the
+ *                     short bottom->top walk runs beside the long horizontal
+ *                     ones instead of after them
+ *   SGM_WTA_AUTO      SEPARATE when the four-direction launch has at most
+ *                     2.5 waves per SIMD (single pairs, where the scanline
+ *                     walks set the time) and no matched image is more than
+ *                     2.5 times as wide as it is high; FUSED otherwise
+ * The on-the-fly kernels and the persistent batch schedules always fuse.
+ * Process-wide, read when the launches are enqueued (a batch's HIP graph keeps
+ * the form it was captured with).  All give byte-identical results. */
+#define SGM_WTA_AUTO 0
+#define SGM_WTA_FUSED 1
+#define SGM_WTA_SEPARATE 2
+int sgm_set_wta_form(int32_t form);
+int32_t sgm_get_wta_form(void);
+
 /* Row strips of the peak removal (do_remove_peaks; StereoCudaImpl.cpp:32-97,
  * 463-498).  The reference splits the left map into
  * hardware_concurrency() * 3 / 4 strips of H / strips rows (rows past the last

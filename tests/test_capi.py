@@ -207,6 +207,16 @@ def test_window_form_setting():
     assert lib.sgm_get_window_form() == 0
 
 
+def test_wta_form_setting():
+    lib = _lib.lib()
+    assert lib.sgm_get_wta_form() == 0             # SGM_WTA_AUTO
+    for form in (1, 2, 0):
+        assert lib.sgm_set_wta_form(form) == 0 and lib.sgm_get_wta_form() == form
+    assert lib.sgm_set_wta_form(3) == _lib.SGM_E_INVALID_ARG
+    assert lib.sgm_set_wta_form(-1) == _lib.SGM_E_INVALID_ARG
+    assert lib.sgm_get_wta_form() == 0
+
+
 def test_uninitialised_handle_and_constants():
     st = StereoCuda()
     import numpy as np

@@ -209,6 +209,34 @@ def test_schedules_give_identical_bytes(cuda, cost_mode, mode, lr):
         b.set_schedule(9)
 
 
+@pytest.mark.parametrize("lr", [True, False])
+def test_wta_forms_give_identical_bytes(cuda, cost_mode, lr):
+    """Stored-cost batches on the grid schedule with bottom->top fused into
+    the WTA and with the four directions followed by the standalone WTA give
+    the same bytes, eagerly and captured in a graph."""
+    from fasterstereocuda_amd import stages
+    from fasterstereocuda_amd.batch import SCHED_GRID
+    cost_mode(1)
+    before = stages.get_wta_form()
+    try:
+        pairs = _uneven_pairs(cuda, lr)
+        b = SgmBatch(pairs, CuSGMOption.benchmark(do_lr_check=lr))
+        b.set_schedule(SCHED_GRID)
+        stages.set_wta_form(stages.WTA_FUSED)
+        b.run()
+        ref = _outputs(pairs)
+        stages.set_wta_form(stages.WTA_SEPARATE)
+        for graph in (False, True):
+            print(f"[wta form] lr={lr} separate, graph={graph}", flush=True)
+            b.use_graph(graph)
+            _clobber(pairs)
+            b.run()
+            for k, (g, r) in enumerate(zip(_outputs(pairs), ref)):
+                np.testing.assert_array_equal(g, r, err_msg=f"graph={graph} map {k}")
+    finally:
+        stages.set_wta_form(before)
+
+
 @pytest.mark.parametrize("mode", [2, 1], ids=["on-the-fly", "stored"])
 @pytest.mark.parametrize("policy", [0, 2])
 def test_graph_replays_match_eager(cuda, cost_mode, mode, policy):

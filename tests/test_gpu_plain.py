@@ -181,6 +181,39 @@ def test_cost_modes_match_oracle(cuda, cost_mode, mode, case, lr, median):
     st.Release()
 
 
+@pytest.fixture
+def wta_form():
+    before = stages.get_wta_form()
+    yield stages.set_wta_form
+    stages.set_wta_form(before)
+
+
+@pytest.mark.parametrize("form", [stages.WTA_FUSED, stages.WTA_SEPARATE], ids=["fused", "separate"])
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+@pytest.mark.parametrize("lr,median", [(True, True), (False, False)])
+def test_wta_forms_match_oracle(cuda, cost_mode, wta_form, form, case, lr, median):
+    """The stored-cost path with bottom->top fused into the WTA and with the
+    four directions followed by the standalone WTA kernel gives the oracle's
+    maps."""
+    W, H, D, min_d, cs, p1, p2 = case
+    print(f"[wta form {form}] {case} lr={lr} median={median}", flush=True)
+    cost_mode(1)
+    wta_form(form)
+    left, right = _pair(W, H, D, seed=7)
+    exp_l, exp_r = oracle.match(left, right, min_d, D, cs_type=cs, p1=p1, p2=p2, uniqueness=UNIQ,
+                                do_lr_check=lr, lr_threshold=LR_THR, do_median=median)
+    st = StereoCuda()
+    opt = CuSGMOption.benchmark(cs_type=cs, p1=p1, p2_init=p2, do_lr_check=lr,
+                                do_median_filter=median)
+    assert st.Init(W, H, min_d, D, opt), st.last_error
+    dl = np.empty((H, W), np.float32)
+    dr = np.empty((H, W), np.float32)
+    assert st.Match(left, right, dl, dr), st.last_error
+    _eq_float(dl, exp_l, "left map")
+    _eq_float(dr, exp_r, "right map")
+    st.Release()
+
+
 def test_driving_full_size(cuda):
     pair = scenes.driving(0)
     print("[match] driving 1242x375", flush=True)
