@@ -370,7 +370,16 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
             wta_step(other, i);
         }
     };
-    auto ring_steps = [&](int i) __attribute__((always_inline)) {   // steps i .. i + PF - 1
+    // Steps i .. i + PF - 1, refilling each slot with step + PF (clamped to
+    // step n - 1 near the end of the line).  Without FUSE the refill offsets
+    // come from one multiply per turn and scalar adds / selects, so each load
+    // adds a scalar offset to the lane's loop-invariant address instead of a
+    // 64-bit multiply-add per load.  The fused kernel keeps the multiply: the
+    // scalar form gives it four 64-bit lane bases and 37 more VGPRs, and its
+    // larger plans run slower at the lower occupancy.
+    const int64_t last_off = (int64_t)(n - 1) * step;
+    auto ring_turn = [&](int i) __attribute__((always_inline)) {
+        int64_t so = (int64_t)(i + PF) * step;
 #pragma unroll
         for (int p = 0; p < PF; ++p) {
             uint32_t cur[NSRC][NW];
@@ -378,9 +387,16 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
             for (int q = 0; q < NSRC; ++q)
 #pragma unroll
                 for (int k = 0; k < NW; ++k) cur[q][k] = buf[p][q][k];
-            const int64_t so = (int64_t)min(i + p + PF, n - 1) * step;
+            int64_t o;
+            if constexpr (FUSE) {
+                o = (int64_t)min(i + p + PF, n - 1) * step;
+            } else {
+                o = (i + p + PF <= n - 1) ? so : last_off;
+                asm volatile("" : "+s"(o));   // keep it a scalar of its own
+            }
 #pragma unroll
-            for (int q = 0; q < NSRC; ++q) load_words<NW>(srcs[q] + so + voff, buf[p][q]);
+            for (int q = 0; q < NSRC; ++q) load_words<NW>(srcs[q] + o + voff, buf[p][q]);
+            so += step;
             consume(cur, i + p);
         }
     };
@@ -390,11 +406,11 @@ __device__ __forceinline__ void agg_lines(const SgmJob& J, int path, int group, 
         constexpr int kUnroll = (D <= 128) ? 16 / PF : 1;
         for (; i + 16 <= n; i += 16) {
 #pragma unroll kUnroll
-            for (int s0 = 0; s0 < 16; s0 += PF) ring_steps(i + s0);
+            for (int s0 = 0; s0 < 16; s0 += PF) ring_turn(i + s0);
             wta_flush(i, 16);
         }
     } else {
-        for (; i + PF <= n; i += PF) ring_steps(i);
+        for (; i + PF <= n; i += PF) ring_turn(i);
     }
     for (; i < n; ++i) {
         uint32_t cur[NSRC][NW];
