@@ -331,7 +331,10 @@ __device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int la
     const int kown = x - c0;
     const int kp = LPW + (right ? (x - c0) + DPL * q : (x - c0) + (D - 1) - DPL * q);
     const uint32_t P1v = (uint32_t)J.p1 * 0x10001u, P2v = (uint32_t)J.p2 * 0x10001u;
-    const int64_t off0 = ((int64_t)y0 * W + x) * D + q * DPL;
+    // volume offsets: a wave-uniform row part and the lane's 32-bit part
+    const int64_t ubase = ((int64_t)y0 * W + c0) * D;
+    const uint32_t loff = (uint32_t)((x - c0) * D + q * DPL);
+    const int64_t off0 = ubase + loff;
     const int64_t ostep = dy * (int64_t)W * D;
     uint8_t* const out_base = FUSE ? J.path[kDU] : J.path[kUD];
 
@@ -464,6 +467,10 @@ __device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int la
             if (active && first) gst(J.disp + (int64_t)y0 * W + x, kInvalidDisp);
         }
     }
+    // row offsets of step i (stores) and of the refill step min(i + PF, H - 1)
+    // (loads), advanced by scalar adds: no 64-bit multiply per access
+    int64_t row_i = ubase + ostep, row_f = ubase + (int64_t)(1 + PF) * ostep;
+    const int64_t row_last = ubase + (int64_t)(H - 1) * ostep;
     for (int i0 = 1; i0 < H; i0 += BLK) {
 #pragma unroll
         for (int b = 0; b < BLK; ++b) {
@@ -478,14 +485,19 @@ __device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int la
             if constexpr (!FUSE) {
                 uint32_t o[NW];
                 pack_lane(s, kSelFwd, 0u, o);
-                if (active && i < H) store_words<NW>(out_base + off0 + (int64_t)i * ostep, o);
+                int64_t r = row_i;
+                asm volatile("" : "+s"(r));   // keep it a scalar of its own
+                if (active && i < H) store_words<NW>(out_base + r + loff, o);
             } else {
                 // consume the slot, then refill it for step i + PF
                 wta_step(s, pr[p], i);
+                int64_t r = (i + PF <= H - 1) ? row_f : row_last;
+                asm volatile("" : "+s"(r));
 #pragma unroll
-                for (int u = 0; u < 3; ++u)
-                    load_words<NW>(srcs[u] + off0 + (int64_t)min(i + PF, H - 1) * ostep, pr[p][u]);
+                for (int u = 0; u < 3; ++u) load_words<NW>(srcs[u] + r + loff, pr[p][u]);
             }
+            row_i += ostep;
+            row_f += ostep;
         }
         if constexpr (FUSE) wta_flush(i0);
     }
