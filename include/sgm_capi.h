@@ -245,7 +245,9 @@ int sgm_batch_run(sgm_batch* b, void* stream);
 int sgm_batch_set_timing(sgm_batch* b, int32_t on);
 /* Sums of the per-stage times (ms) of the runs recorded since the last call:
  * ms[0..4] = census, cost, aggregate (left->right, right->left, top->bottom),
- * aggregate bottom->top fused with the WTA, post.  Waits for those runs. */
+ * aggregate bottom->top fused with the WTA, post.  With the separate WTA form
+ * (sgm_set_wta_form) ms[2] covers all four directions and ms[3] the WTA.
+ * Waits for those runs. */
 int sgm_batch_stage_times(sgm_batch* b, float* ms, int32_t* runs);
 size_t sgm_batch_device_bytes(const sgm_batch* b);
 void sgm_batch_destroy(sgm_batch* b);
