@@ -100,12 +100,23 @@ __global__ __launch_bounds__(256) void cost_init_kernel(PyrLevel a) {
     uint32_t words[LR / 4];
 #pragma unroll
     for (int k = 0; k < LR / 4; ++k) words[k] = 0u;
+    if (x - b - (LR - 1) > 0 && x - b < a.w) {
+        // every partner column of the window is inside the image (all but
+        // the pixels near the borders and those with an invalid base)
+        const uint64_t* crb = cr + (x - b);
 #pragma unroll
-    for (int t = 0; t < LR; ++t) {
-        const int xr = x - t - b;
-        uint32_t c = invalid;
-        if (xr > 0 && xr < a.w) c = (uint32_t)__popcll(cl ^ cr[xr]) + cost_adjust(t, LR);
-        words[t >> 2] |= (c & 0xFFu) << (8 * (t & 3));
+        for (int t = 0; t < LR; ++t) {
+            const uint32_t c = (uint32_t)__popcll(cl ^ crb[-t]) + cost_adjust(t, LR);
+            words[t >> 2] |= (c & 0xFFu) << (8 * (t & 3));
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < LR; ++t) {
+            const int xr = x - t - b;
+            uint32_t c = invalid;
+            if (xr > 0 && xr < a.w) c = (uint32_t)__popcll(cl ^ cr[xr]) + cost_adjust(t, LR);
+            words[t >> 2] |= (c & 0xFFu) << (8 * (t & 3));
+        }
     }
     uint4* dst = reinterpret_cast<uint4*>(a.cost + i * LR);
 #pragma unroll
