@@ -32,43 +32,81 @@ using namespace dev;
 // ---------------------------------------------------------------- census ---
 // cusgm_cc.cu:30-106.  Bit = (neighbour < centre), MSB-first raster order,
 // centre included; border pixels get 0 (left view) or 0xFFFFFFFF (right).
-// The thread's 4 vertically adjacent pixels share the window rows they
-// overlap in: 4 + 2 RH tile rows of 2 RW + 1 bytes are read once.  A bit is
-// two VALU ops: d = neighbour - centre (as u32, negative iff neighbour <
-// centre) and acc = alignbit(acc, d, 31) = (acc << 1) | (d >> 31).  The first
-// NB - 32 bits (NB = window size) go to the high word, the last 32 to the low.
+// A thread computes two horizontally adjacent pixels as the u16 halves of one
+// register, for 4 vertically adjacent rows that share the window rows they
+// overlap in.  The tile is staged in LDS as overlapping pairs, P[r][c] =
+// (I[r][c], I[r][c + 1]), so one dword read gives a window column for both
+// pixels.  Per window bit and pixel pair: d = P - centre pair (v_pk_sub_u16,
+// bit 15 of each half set iff neighbour < centre), then acc = (acc >> 1) |
+// (d & 0x80008000): the sign bits enter at the top of each half.  The bits
+// run from the last to the first, 16 per accumulator, so each half ends up
+// holding 16 consecutive bits of its pixel's word in MSB-first order (a
+// half never holds more than 16, so the 32-bit shift never carries a bit
+// from one half into the other).  The 64-bit word of a pixel: its last 32
+// bits in the low word, the first NB - 32 in the high word.
+// (acc >> 1) | (d & m) as exactly v_lshrrev + v_and_or: written in C, the
+// compiler distributes the shifts of a chain of these over its ORs and
+// issues about twice the instructions.
+__device__ __forceinline__ uint32_t shift_in(uint32_t acc, uint32_t d, uint32_t m) {
+    uint32_t r;
+    asm("v_lshrrev_b32 %0, 1, %1\n\t"
+        "v_and_or_b32 %0, %2, %3, %0"
+        : "=&v"(r)
+        : "v"(acc), "v"(d), "s"(m));
+    return r;
+}
+
 template <int RW, int RH>
-__device__ __forceinline__ void census_column4(const uint8_t (*t)[72], int r0, int c,
-                                               uint64_t (&out)[4]) {
+__device__ __forceinline__ void census_pairs4(const uint32_t (*t)[72], int r0, int c,
+                                              uint64_t (&out)[4][2]) {
     constexpr int NJ = 2 * RW + 1, NB = NJ * (2 * RH + 1);
-    uint32_t hi[4] = {0u, 0u, 0u, 0u}, lo[4] = {0u, 0u, 0u, 0u}, ctr[4];
+    constexpr int NC = (NB + 15) / 16;   // 16-bit chunks per pixel
+    uint32_t acc[4][4], ctr[4];
 #pragma unroll
-    for (int p = 0; p < 4; ++p) ctr[p] = t[r0 + p][c];
+    for (int p = 0; p < 4; ++p) {
+        ctr[p] = t[r0 + p][c];
 #pragma unroll
-    for (int k = 0; k < 4 + 2 * RH; ++k) {      // tile rows r0 - RH .. r0 + 3 + RH
+        for (int q = 0; q < 4; ++q) acc[p][q] = 0u;
+    }
+#pragma unroll
+    for (int k = 3 + 2 * RH; k >= 0; --k) {       // tile rows r0 - RH .. r0 + 3 + RH, last first
         uint32_t nb[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; ++j) nb[j] = t[r0 - RH + k][c - RW + j];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-            const int i = k - p;                 // window row of pixel p
+            const int i = k - p;                  // window row of pixel p
             if (i < 0 || i > 2 * RH) continue;
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) {
-                const uint32_t d = nb[j] - ctr[p];
-                if (i * NJ + j < NB - 32) hi[p] = __builtin_amdgcn_alignbit(hi[p], d, 31);
-                else lo[p] = __builtin_amdgcn_alignbit(lo[p], d, 31);
+            for (int j = NJ - 1; j >= 0; --j) {
+                const int q = (NB - 1 - (i * NJ + j)) / 16;   // chunk of this bit
+                const uint32_t d = pk_sub(nb[j], ctr[p]);   // bit 15 of a half set iff nb < ctr
+                acc[p][q] = shift_in(acc[p][q], d, 0x80008000u);
             }
         }
     }
 #pragma unroll
-    for (int p = 0; p < 4; ++p) out[p] = ((uint64_t)hi[p] << 32) | lo[p];
+    for (int p = 0; p < 4; ++p) {
+        // a partial last chunk: move its bits down to the bottom of each half
+        constexpr int last = NB - 16 * (NC - 1);
+        if constexpr (last < 16) acc[p][NC - 1] >>= (16 - last);
+        // chunks 0, 1: the low word; chunks 2, 3: the high word
+        const uint32_t lo0 = __builtin_amdgcn_perm(acc[p][1], acc[p][0], 0x05040100u);
+        const uint32_t lo1 = __builtin_amdgcn_perm(acc[p][1], acc[p][0], 0x07060302u);
+        const uint32_t hi0 = __builtin_amdgcn_perm(acc[p][3], acc[p][2], 0x05040100u);
+        const uint32_t hi1 = __builtin_amdgcn_perm(acc[p][3], acc[p][2], 0x07060302u);
+        out[p][0] = ((uint64_t)hi0 << 32) | lo0;
+        out[p][1] = ((uint64_t)hi1 << 32) | lo1;
+    }
 }
 
-__global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ jobs,
+// 64 x 16 pixels per workgroup of 128 threads: thread (lx, ly) takes columns
+// x0 + 2 lx, x0 + 2 lx + 1 and rows y0 + 4 ly .. y0 + 4 ly + 3.
+__global__ __launch_bounds__(128) void census_kernel(const SgmJob* __restrict__ jobs,
                                                      const int* __restrict__ offs, int njobs,
                                                      IoBind io) {
     __shared__ uint8_t tile[22][72];
+    __shared__ uint32_t pairs[22][72];
     const int item = blockIdx.x;
     const int j = find_job(offs, njobs, item);
     const SgmJob& J = jobs[j];
@@ -77,13 +115,13 @@ __global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ 
     const int tiles_x = (J.W + 63) / 64;
     const int x0 = (local % tiles_x) * 64, y0 = (local / tiles_x) * 16;
     const int tid = threadIdx.x;
-    // 22 x 72 bytes over 256 threads: every load is issued before the first
-    // LDS store, so the 7 loads of a thread overlap
-    constexpr int kTileN = 22 * 72, kPer = (kTileN + 255) / 256;
+    // 22 x 72 bytes over 128 threads: every load is issued before the first
+    // LDS store, so the loads of a thread overlap
+    constexpr int kTileN = 22 * 72, kPer = (kTileN + 127) / 128;
     uint8_t px[kPer];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-        const int idx = tid + 256 * u;
+        const int idx = tid + 128 * u;
         const int r = idx / 72, c = idx % 72;
         const int yy = min(max(y0 - 3 + r, 0), J.H - 1);
         const int xx = min(max(x0 - 4 + c, 0), J.W - 1);
@@ -91,22 +129,35 @@ __global__ __launch_bounds__(256) void census_kernel(const SgmJob* __restrict__ 
     }
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-        const int idx = tid + 256 * u;
+        const int idx = tid + 128 * u;
         if (idx < kTileN) tile[idx / 72][idx % 72] = px[u];
     }
     __syncthreads();
-    const int lx = tid & 63, ly = tid >> 6;
+    // the overlapping pairs (column 71 has no right neighbour and is not read)
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int idx = tid + 128 * u;
+        const int r = idx / 72, c = idx % 72;
+        if (idx < kTileN && c < 71) pairs[r][c] = (uint32_t)tile[r][c] | ((uint32_t)tile[r][c + 1] << 16);
+    }
+    __syncthreads();
+    const int lx = tid & 31, ly = tid >> 5;
     const int rw = J.cs_type ? 4 : 2, rh = J.cs_type ? 3 : 2;
     const uint64_t invalid = J.is_right ? 0xFFFFFFFFull : 0ull;
-    uint64_t v[4];
-    if (J.cs_type) census_column4<4, 3>(tile, ly * 4 + 3, lx + 4, v);
-    else census_column4<2, 2>(tile, ly * 4 + 3, lx + 4, v);
+    uint64_t v[4][2];
+    if (J.cs_type) census_pairs4<4, 3>(pairs, ly * 4 + 3, 2 * lx + 4, v);
+    else census_pairs4<2, 2>(pairs, ly * 4 + 3, 2 * lx + 4, v);
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
-        const int y = y0 + ly * 4 + rr, x = x0 + lx;
-        if (x >= J.W || y >= J.H) continue;
-        const bool border = x < rw || x >= J.W - rw || y < rh || y >= J.H - rh;
-        gst(J.census + (int64_t)y * J.W + x, border ? invalid : v[rr]);
+        const int y = y0 + ly * 4 + rr;
+        if (y >= J.H) continue;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int x = x0 + 2 * lx + h;
+            if (x >= J.W) continue;
+            const bool border = x < rw || x >= J.W - rw || y < rh || y >= J.H - rh;
+            gst(J.census + (int64_t)y * J.W + x, border ? invalid : v[rr][h]);
+        }
     }
 }
 
@@ -747,7 +798,7 @@ hipError_t launch_depth(const float* disp, float* depth, size_t n, double f1, do
 
 hipError_t launch_census(const StagePlan& p, hipStream_t s) {
     if (p.n_census <= 0) return hipSuccess;
-    hipLaunchKernelGGL(census_kernel, dim3(p.n_census), dim3(256), 0, s, p.jobs, p.offs_census,
+    hipLaunchKernelGGL(census_kernel, dim3(p.n_census), dim3(128), 0, s, p.jobs, p.offs_census,
                        p.njobs, p.io);
     return hipGetLastError();
 }
