@@ -63,7 +63,7 @@ inline int agg_items(const SgmJob& j) { return 2 * ((j.H + 3) / 4) + 2 * ((j.W -
 inline int agg3_items(const SgmJob& j) { return 2 * ((j.H + 3) / 4) + ((j.W - j.roi_x + 3) / 4); }
 inline int dw_items(const SgmJob& j) { return (j.W - j.roi_x + 3) / 4; }
 inline int wta_items(const SgmJob& j) { return j.H * ((j.W + 15) / 16); }
-inline int post_items(const SgmPost& p) { return ((p.W + 15) / 16) * ((p.H + 15) / 16); }
+inline int post_items(const SgmPost& p) { return ((p.W + 63) / 64) * ((p.H + 15) / 16); }
 // On-the-fly cost aggregation (sgm_wide.hip).  Each kernel holds DPL
 // disparities per lane, so a scanline takes D / DPL lanes (at most one
 // 16-lane DPP row) and a wave covers 64 * DPL / D scanlines.

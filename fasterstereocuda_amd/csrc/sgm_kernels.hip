@@ -659,10 +659,16 @@ __device__ __forceinline__ float median9(float a0, float a1, float a2, float b0,
     return __builtin_amdgcn_fmed3f(p, q, r);
 }
 
+// 64 x 16 pixels per workgroup of 256 threads.  The tile with its one-pixel
+// halo (66 x 18 LR-checked values) is built in LDS first: a thread issues
+// its 5 left-map loads, then the 5 dependent right-map loads, so a workgroup
+// waits for two load latencies in all.  Each thread then takes a column of 4
+// pixels, whose 3 x 3 windows share the LDS rows they overlap in.
+constexpr int kPostTW = 66, kPostTH = 18;
 __global__ __launch_bounds__(256) void post_kernel(const SgmPost* __restrict__ posts,
                                                    const int* __restrict__ offs, int nposts,
                                                    IoBind io) {
-    __shared__ float t[18][18];
+    __shared__ float t[kPostTH][kPostTW];
     const int item = blockIdx.x;
     const int j = find_job(offs, nposts, item);
     const SgmPost& P = posts[j];
@@ -670,45 +676,56 @@ __global__ __launch_bounds__(256) void post_kernel(const SgmPost* __restrict__ p
     float* const out_l = bound ? io.out_l : P.out_l;
     float* const out_r = bound ? io.out_r : P.out_r;
     const int local = item - offs[j];
-    const int tiles_x = (P.W + 15) / 16;
-    const int x0 = (local % tiles_x) * 16, y0 = (local / tiles_x) * 16;
+    const int tiles_x = (P.W + 63) / 64;
+    const int x0 = (local % tiles_x) * 64, y0 = (local / tiles_x) * 16;
     const int tid = threadIdx.x;
-    // two tile entries per thread (18 x 18 = 256 + 68): both left-map loads
-    // are issued before either right-map load that depends on them
-    float dl[2];
-    int ex[2], ey[2];
-    bool in[2];
+    constexpr int kN = kPostTW * kPostTH, kPer = (kN + 255) / 256;
+    float dl[kPer];
+    int ex[kPer], ey[kPer];
+    bool in[kPer];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kPer; ++u) {
         const int idx = tid + 256 * u;
-        ex[u] = x0 - 1 + idx % 18;
-        ey[u] = y0 - 1 + idx / 18;
-        in[u] = idx < 18 * 18 && ex[u] >= 0 && ey[u] >= 0 && ex[u] < P.W && ey[u] < P.H;
+        ex[u] = x0 - 1 + idx % kPostTW;
+        ey[u] = y0 - 1 + idx / kPostTW;
+        in[u] = idx < kN && ex[u] >= 0 && ey[u] >= 0 && ex[u] < P.W && ey[u] < P.H;
         dl[u] = in[u] ? raw_at(P.raw_l, P, ex[u], ey[u]) : kInvalidDisp;
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kPer; ++u) {
         const int idx = tid + 256 * u;
-        if (idx < 18 * 18) t[idx / 18][idx % 18] = in[u] ? lr_finish(P, ex[u], ey[u], dl[u]) : kInvalidDisp;
+        if (idx < kN)
+            t[idx / kPostTW][idx % kPostTW] = in[u] ? lr_finish(P, ex[u], ey[u], dl[u]) : kInvalidDisp;
     }
     __syncthreads();
-    const int tx = tid & 15, ty = tid >> 4;
-    const int x = x0 + tx, y = y0 + ty;
-    if (x >= P.W || y >= P.H) return;
-    // the right output starts as the right map (LR invalidations follow in a
-    // second launch) or, without the LR check, as all INVALID
-    if (out_r)
-        gst(out_r + (int64_t)y * P.W + x, P.do_lr ? raw_at(P.raw_r, P, x, y) : kInvalidDisp);
-    float v = t[ty + 1][tx + 1];
-    if (P.do_median && x > 0 && y > 0)
-        v = median9(t[ty][tx], t[ty][tx + 1], t[ty][tx + 2], t[ty + 1][tx], t[ty + 1][tx + 1],
-                    t[ty + 1][tx + 2], t[ty + 2][tx], t[ty + 2][tx + 1], t[ty + 2][tx + 2]);
-    gst(out_l + (int64_t)y * P.W + x, v);
+    const int tx = tid & 63, ty = (tid >> 6) * 4;
+    const int x = x0 + tx;
+    if (x >= P.W) return;
+    float c[6][3];   // tile rows ty .. ty + 5, columns tx .. tx + 2
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c[r][k] = t[ty + r][tx + k];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int y = y0 + ty + r;
+        if (y >= P.H) break;
+        // the right output starts as the right map (LR invalidations follow
+        // in a second launch) or, without the LR check, as all INVALID
+        if (out_r)
+            gst(out_r + (int64_t)y * P.W + x, P.do_lr ? raw_at(P.raw_r, P, x, y) : kInvalidDisp);
+        float v = c[r + 1][1];
+        if (P.do_median && x > 0 && y > 0)
+            v = median9(c[r][0], c[r][1], c[r][2], c[r + 1][0], c[r + 1][1], c[r + 1][2],
+                        c[r + 2][0], c[r + 2][1], c[r + 2][2]);
+        gst(out_l + (int64_t)y * P.W + x, v);
+    }
 }
 
 // Right-map invalidation of the LR check (cusgm_ca.cu:3894-3895): every left
 // pixel that fails marks its partner INVALID in out_r, which holds a copy of
 // the right map.  All writers store the same value, so the result is fixed.
+// Same 64 x 16 tiles as post_kernel, a column of 4 pixels per thread.
 __global__ __launch_bounds__(256) void right_invalidate_kernel(const SgmPost* __restrict__ posts,
                                                                const int* __restrict__ offs,
                                                                int nposts, IoBind io) {
@@ -718,17 +735,24 @@ __global__ __launch_bounds__(256) void right_invalidate_kernel(const SgmPost* __
     float* const out_r = (io.on && j == 0) ? io.out_r : P.out_r;
     if (!out_r || !P.do_lr) return;
     const int local = item - offs[j];
-    const int tiles_x = (P.W + 15) / 16;
-    const int x = (local % tiles_x) * 16 + (threadIdx.x & 15);
-    const int y = (local / tiles_x) * 16 + (threadIdx.x >> 4);
-    if (x >= P.W || y >= P.H) return;
-    const float dl = raw_at(P.raw_l, P, x, y);
-    if (dl == kInvalidDisp) return;
-    const float xrf = (float)x - dl;
-    const int xr = (xrf > 0) ? (int)((double)xrf + 0.5) : (int)((double)xrf - 0.5);
-    if (xr < 0 || xr >= P.W) return;
-    const float dr = raw_at(P.raw_r, P, xr, y);
-    if (fabsf(dl - dr) > P.lr_thr) gst(out_r + (int64_t)y * P.W + xr, kInvalidDisp);
+    const int tiles_x = (P.W + 63) / 64;
+    const int x = (local % tiles_x) * 64 + (threadIdx.x & 63);
+    const int yb = (local / tiles_x) * 16 + (threadIdx.x >> 6) * 4;
+    if (x >= P.W) return;
+    float dls[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dls[r] = (yb + r < P.H) ? raw_at(P.raw_l, P, x, yb + r) : kInvalidDisp;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int y = yb + r;
+        const float dl = dls[r];
+        if (y >= P.H || dl == kInvalidDisp) continue;
+        const float xrf = (float)x - dl;
+        const int xr = (xrf > 0) ? (int)((double)xrf + 0.5) : (int)((double)xrf - 0.5);
+        if (xr < 0 || xr >= P.W) continue;
+        const float dr = raw_at(P.raw_r, P, xr, y);
+        if (fabsf(dl - dr) > P.lr_thr) gst(out_r + (int64_t)y * P.W + xr, kInvalidDisp);
+    }
 }
 
 // Depth from disparity (cusgm_util.cu:7-33): f1 = baseline * focus and
