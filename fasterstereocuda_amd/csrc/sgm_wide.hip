@@ -98,8 +98,14 @@ __device__ __forceinline__ uint32_t popc_acc(uint64_t a, uint32_t acc) {
 template <int NP>
 struct Line {
     uint32_t L[NP];
-    uint32_t m;
+    uint32_t m;   // the line minimum in both u16 halves
 };
+
+// both halves = the smaller half (v_pk_min_u16 with swapped operand halves);
+// a u32 minimum of such values is their u16 minimum
+__device__ __forceinline__ uint32_t pk_min_halves(uint32_t v) {
+    return as_u(__builtin_elementwise_min(as_v(v), as_v(v).yx));
+}
 
 template <int LPL, int NP>
 __device__ __forceinline__ void line_init(Line<NP>& s, const uint32_t (&c)[NP]) {
@@ -109,13 +115,13 @@ __device__ __forceinline__ void line_init(Line<NP>& s, const uint32_t (&c)[NP]) 
         s.L[k] = c[k];
         t = pk_min(t, c[k]);
     }
-    s.m = line_min<LPL>(min(t & 0xFFFFu, t >> 16));
+    s.m = line_min<LPL>(pk_min_halves(t));
 }
 
 template <int LPL, int NP>
 __device__ __forceinline__ void line_step(Line<NP>& s, const uint32_t (&c)[NP], uint32_t P1v,
                                           uint32_t P2v, bool first, bool last) {
-    const uint32_t mv = s.m * 0x10001u;
+    const uint32_t mv = s.m;
     const uint32_t mp2 = pk_add(mv, P2v);
     uint32_t R[NP];
 #pragma unroll
@@ -140,7 +146,7 @@ __device__ __forceinline__ void line_step(Line<NP>& s, const uint32_t (&c)[NP], 
     for (int w = 1; w < NP; w *= 2)
 #pragma unroll
         for (int k = 0; k + w < NP; k += 2 * w) T[k] = pk_min(T[k], T[k + w]);
-    s.m = line_min<LPL>(min(T[0] & 0xFFFFu, T[0] >> 16));
+    s.m = line_min<LPL>(pk_min_halves(T[0]));
 }
 
 // the lane's 2 NP bytes (uint8 truncation) in memory order; `sel` reverses
@@ -402,13 +408,20 @@ __device__ __forceinline__ void wide_vertical(const SgmJob& J, int group, int la
             const uint32_t a2 = (k & 1) ? bytes23(o3[2][w]) : bytes01(o3[2][w]);
             S[k] = a0 + a1 + a2 + (s.L[k] & 0x00FF00FFu);   // each half <= 1020
         }
-        uint32_t best = 0xFFFFFFFFu;
+        // argmin with the smallest d: inside the lane on u16 keys S * DPL + j
+        // (S <= 4 * 255 < 2^10 and j < DPL <= 64: the keys fit in 16 bits, so
+        // one 32-bit v_lshl_or_b32 per pair shifts both halves without a carry
+        // between them), then across the line's lanes on (S << 16) | d
+        static_assert(DPL <= 64 && (DPL & (DPL - 1)) == 0, "u16 argmin keys S * DPL + j");
+        constexpr int kLog = __builtin_ctz(DPL);
+        uint32_t key = 0xFFFFFFFFu;
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
-            const uint32_t d0 = (uint32_t)(DPL * q + 2 * k);
-            best = min(best, (S[k] << 16) | d0);
-            best = min(best, (S[k] & 0xFFFF0000u) | (d0 + 1));
+            const uint32_t jk = (uint32_t)(2 * k) | ((uint32_t)(2 * k + 1) << 16);
+            key = pk_min(key, (S[k] << kLog) | jk);
         }
+        key = pk_min_halves(key) & 0xFFFFu;
+        uint32_t best = ((key / DPL) << 16) | (uint32_t)(DPL * q + key % DPL);
         best = line_min<LPL>(best);
         const int ds = (int)(best & 0xFFFFu);
         // S(ds - 1) and S(ds + 1): the lanes of a line hold consecutive
