@@ -310,19 +310,24 @@ sgm_pyramid* sgm_pyramid_create2(int32_t width, int32_t height, float min_depth,
 
 namespace {
 
-// Pipeline over the level-0 images already in p->il[0] / p->ir[0]; the
-// result is left in p->out[0].
-int run_pyramid(sgm_pyramid* p, hipStream_t s) {
+// Pipeline over the level-0 images in p->il[0] / p->ir[0], the result left
+// in p->out[0]; or, with `io`, over the caller's device images io->img[0..1]
+// with the result written to io->out_l (no staging copies).
+int run_pyramid(sgm_pyramid* p, hipStream_t s, const IoBind* io = nullptr) {
     const int top = p->layers - 1;
     for (int i = 1; i <= top; ++i) {
-        PyrZoom z{p->il[i - 1], p->ir[i - 1], p->il[i], p->ir[i], p->lw[i - 1], p->lh[i - 1]};
+        const uint8_t* srcl = (io && i == 1) ? io->img[0] : p->il[i - 1];
+        const uint8_t* srcr = (io && i == 1) ? io->img[1] : p->ir[i - 1];
+        PyrZoom z{srcl, srcr, p->il[i], p->ir[i], p->lw[i - 1], p->lh[i - 1]};
         PYR_TRY(pyr_launch_zoom(z, s));
     }
-    PYR_TRY(launch_match(p->top_plan, false, s, nullptr));
+    StagePlan top_plan = p->top_plan;
+    if (io && top == 0) top_plan.io = *io;
+    PYR_TRY(launch_match(top_plan, false, s, nullptr));
     // the coarsest level of a multi-level pyramid is dilated
     // (FasterStereoCuda.cpp:192-197, cusgm_df.cu:300-318); its peaks are then
     // removed if asked, after the filters as in StereoCudaImpl::Match (:541-542)
-    float* top_map = p->out[top];
+    float* top_map = (io && top == 0) ? io->out_l : p->out[top];
     if (top > 0) {
         const int w = p->lw[top], h = p->lh[top];
         const int sm = std::min(w, h);
@@ -335,7 +340,9 @@ int run_pyramid(sgm_pyramid* p, hipStream_t s) {
     for (int i = top - 1; i >= 0; --i) {
         PyrInit in{top > 0 && i == top - 1 ? p->raw[top] : p->out[i + 1], p->init[i], p->lw[i], p->lh[i], p->LR};
         PYR_TRY(pyr_launch_init_next(in, s));
-        PYR_TRY(launch_census(p->lvl_plan[i], s));
+        StagePlan lp = p->lvl_plan[i];
+        if (io && i == 0) lp.io = *io;
+        PYR_TRY(launch_census(lp, s));
         PyrLevel a{};
         a.census_l = p->cl[i]; a.census_r = p->cr[i]; a.init = p->init[i];
         a.cost = p->cost[i];
@@ -347,7 +354,7 @@ int run_pyramid(sgm_pyramid* p, hipStream_t s) {
         PYR_TRY(pyr_launch_cost_init(a, s));
         PYR_TRY(pyr_launch_aggregate_init(a, s));
         PYR_TRY(pyr_launch_wta_init(a, s));
-        PYR_TRY(launch_post(p->lvl_plan[i], s));
+        PYR_TRY(launch_post(lp, s));
     }
     return SGM_OK;
 }
@@ -373,12 +380,15 @@ int sgm_pyramid_match_device(sgm_pyramid* p, const uint8_t* d_left, const uint8_
         return set_global_error(SGM_E_INVALID_ARG, "pyramid: NULL argument");
     PYR_TRY(hipSetDevice(p->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const size_t npix0 = (size_t)p->W * p->H;
-    PYR_TRY(hipMemcpyAsync(p->il[0], d_left, npix0, hipMemcpyDeviceToDevice, s));
-    PYR_TRY(hipMemcpyAsync(p->ir[0], d_right, npix0, hipMemcpyDeviceToDevice, s));
-    if (int rc = run_pyramid(p, s)) return rc;
-    PYR_TRY(hipMemcpyAsync(d_disp, p->out[0], npix0 * 4, hipMemcpyDeviceToDevice, s));
-    return SGM_OK;
+    // the caller's buffers are bound in place of the level-0 image and map
+    // buffers: nothing is staged
+    IoBind io{};
+    io.img[0] = d_left;
+    io.img[1] = d_right;
+    io.out_l = d_disp;
+    io.out_r = nullptr;
+    io.on = 1;
+    return run_pyramid(p, s, &io);
 }
 
 int sgm_pyramid_match2_device(sgm_pyramid* p, const uint8_t* d_left, const uint8_t* d_right,
@@ -390,9 +400,13 @@ int sgm_pyramid_match2_device(sgm_pyramid* p, const uint8_t* d_left, const uint8
     PYR_TRY(hipSetDevice(p->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t npix0 = (size_t)p->W * p->H;
-    PYR_TRY(hipMemcpyAsync(p->il[0], d_left, npix0, hipMemcpyDeviceToDevice, s));
-    PYR_TRY(hipMemcpyAsync(p->ir[0], d_right, npix0, hipMemcpyDeviceToDevice, s));
-    if (int rc = run_pyramid(p, s)) return rc;
+    IoBind io{};
+    io.img[0] = d_left;
+    io.img[1] = d_right;
+    io.out_l = p->out[0];
+    io.out_r = nullptr;
+    io.on = 1;
+    if (int rc = run_pyramid(p, s, &io)) return rc;
     PYR_TRY(launch_depth(p->out[0], d_depth, npix0, p->cam.baseline * p->cam.focus,
                          p->cam.x0_right - p->cam.x0_left, s));
     return SGM_OK;

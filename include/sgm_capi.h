@@ -323,7 +323,9 @@ sgm_pyramid* sgm_pyramid_create(int32_t width, int32_t height, int32_t min_disp,
                                 int32_t num_layers, int32_t level_range, const sgm_option* opt);
 /* host pointers, blocking */
 int sgm_pyramid_match(sgm_pyramid* p, const uint8_t* left, const uint8_t* right, float* disp);
-/* device pointers, enqueued on stream */
+/* device pointers, enqueued on stream; the finest level reads d_left /
+ * d_right and writes d_disp in place (nothing is staged), so they must stay
+ * valid until the stream reaches the end of the call's work */
 int sgm_pyramid_match_device(sgm_pyramid* p, const uint8_t* d_left, const uint8_t* d_right,
                              float* d_disp, void* stream);
 /* Replaces HierSgmCudaImpl::Init2 (FasterStereoCuda.cpp:297-406): as

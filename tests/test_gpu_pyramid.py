@@ -180,6 +180,32 @@ def test_pyramid_disparity_interval_matches_oracle(cuda, W, H, dmin, dmax, layer
     assert (got != oracle.INVALID).mean() > 0.2
 
 
+@pytest.mark.parametrize("layers,peaks", [(3, False), (2, True), (1, False), (1, True)])
+def test_pyramid_device_entry_binds_caller_buffers(cuda, layers, peaks):
+    """match_device reads the caller's images and writes the caller's map in
+    place (no staging copies): the same bytes as the host entry point and the
+    oracle, including a single level whose map also goes through peak removal,
+    and with the caller's buffers reused for a second pair."""
+    W, H, D, lr = 320, 240, 128, 16
+    print(f"[pyramid] device entry layers{layers} peaks={peaks}", flush=True)
+    opt = CuSGMOption.benchmark(do_remove_peaks=peaks)
+    pyr = SgmPyramid(W, H, 0, D, layers, lr, opt)
+    L = torch.empty((H, W), dtype=torch.uint8, device=cuda)
+    R = torch.empty_like(L)
+    out = torch.full((H, W), 7.0, dtype=torch.float32, device=cuda)
+    for seed in (3, 4):
+        p = scenes.cone_like(seed, H=H, W=W, D=D, workload=2)
+        L.copy_(torch.from_numpy(p.left))
+        R.copy_(torch.from_numpy(p.right))
+        pyr.match_device(L, R, out)
+        torch.cuda.synchronize()
+        host = pyr.match(p.left, p.right)
+        np.testing.assert_array_equal(_bits(out.cpu().numpy()), _bits(host))
+        if not peaks:
+            exp = oracle.pyramid(p.left, p.right, 0, D, layers, lr)
+            np.testing.assert_array_equal(_bits(host), _bits(exp))
+
+
 def test_building_4800x3409_pyramid(cuda):
     """The large-image workload: 4800x3409, D = 256, 3 levels, window 16."""
     p = scenes.building(0)
