@@ -30,20 +30,6 @@ namespace {
 using namespace dev;
 
 // ---------------------------------------------------------------- census ---
-// cusgm_cc.cu:30-106.  Bit = (neighbour < centre), MSB-first raster order,
-// centre included; border pixels get 0 (left view) or 0xFFFFFFFF (right).
-// A thread computes two horizontally adjacent pixels as the u16 halves of one
-// register, for 4 vertically adjacent rows that share the window rows they
-// overlap in.  The tile is staged in LDS as overlapping pairs, P[r][c] =
-// (I[r][c], I[r][c + 1]), so one dword read gives a window column for both
-// pixels.  Per window bit and pixel pair: d = P - centre pair (v_pk_sub_u16,
-// bit 15 of each half set iff neighbour < centre), then acc = (acc >> 1) |
-// (d & 0x80008000): the sign bits enter at the top of each half.  The bits
-// run from the last to the first, 16 per accumulator, so each half ends up
-// holding 16 consecutive bits of its pixel's word in MSB-first order (a
-// half never holds more than 16, so the 32-bit shift never carries a bit
-// from one half into the other).  The 64-bit word of a pixel: its last 32
-// bits in the low word, the first NB - 32 in the high word.
 // (acc >> 1) | (d & m) as exactly v_lshrrev + v_and_or: written in C, the
 // compiler distributes the shifts of a chain of these over its ORs and
 // issues about twice the instructions.
@@ -56,6 +42,20 @@ __device__ __forceinline__ uint32_t shift_in(uint32_t acc, uint32_t d, uint32_t 
     return r;
 }
 
+// cusgm_cc.cu:30-106.  Bit = (neighbour < centre), MSB-first raster order,
+// centre included; border pixels get 0 (left view) or 0xFFFFFFFF (right).
+// A thread computes two horizontally adjacent pixels as the u16 halves of one
+// register, for 4 vertically adjacent rows that share the window rows they
+// overlap in.  The tile is staged in LDS as overlapping pairs, P[r][c] =
+// (I[r][c], I[r][c + 1]), so one dword read gives a window column for both
+// pixels.  Per window bit and pixel pair: d = P - centre pair (v_pk_sub_u16,
+// bit 15 of each half set iff neighbour < centre), then acc = (acc >> 1) |
+// (d & 0x80008000): the compare bits enter at the top of each half.  The bits
+// run from the last to the first, 16 per accumulator, so each half ends up
+// holding 16 consecutive bits of its pixel's word in MSB-first order (a
+// half never holds more than 16, so the 32-bit shift never carries a bit
+// from one half into the other).  The 64-bit word of a pixel: its last 32
+// bits in the low word, the first NB - 32 in the high word.
 template <int RW, int RH>
 __device__ __forceinline__ void census_pairs4(const uint32_t (*t)[72], int r0, int c,
                                               uint64_t (&out)[4][2]) {
