@@ -292,6 +292,43 @@ __global__ __launch_bounds__(64) void agg_init_kernel(PyrLevel a) {
 // representatives, and the stored low bytes agree.  Other penalties take
 // agg_init_kernel.
 typedef unsigned short pk16 __attribute__((ext_vector_type(2)));
+typedef unsigned int pyr_u32x4 __attribute__((ext_vector_type(4)));
+
+// Block stores of the window aggregation: one LR-byte window per pixel, as
+// buffer stores from a base that moves by a fixed step per block, each pixel
+// of a block at an offset from it fixed for the whole walk.  A pixel past the
+// scanlines or the line's end gets an offset beyond the resource's range and
+// its store is dropped, so every block issues the same stores with no branch
+// around them: the compiler then counts the prefetch ring's outstanding loads
+// exactly instead of draining every load and store at each block.  Block blk
+// of scanline group l0 holds pixel (scanline l0 + ln, step blk K + j); the
+// base is the lowest such pixel, offsets stay below (64 + K) W LR bytes.
+template <int LR, int K>
+struct BlockStores {
+    int64_t base0, step;   // base pixel at block 0, and its move per block
+    __device__ BlockStores(bool horiz, bool fwd, int W, int n, int l0) {
+        base0 = horiz ? (int64_t)l0 * W + (fwd ? 0 : n - K) : (fwd ? 0 : (int64_t)(n - K) * W) + l0;
+        step = (horiz ? 1 : (int64_t)W) * (fwd ? K : -K);
+    }
+    // byte offset of (ln, j) from the base
+    __device__ static uint32_t offset(bool horiz, bool fwd, int W, int ln, int j) {
+        const int jj = fwd ? j : K - 1 - j;
+        return (uint32_t)((horiz ? ln * W + jj : jj * W + ln) * LR);
+    }
+    __device__ void store(uint8_t* out, int blk, uint32_t off, bool ok, const uint8_t* src) const {
+        // the base of a block past the line's end may lie before the volume;
+        // none of its stores is issued in range
+        const uintptr_t base = (uintptr_t)out + (uintptr_t)((base0 + blk * step) * LR);
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, 0x7fffffff, 0x00020000);
+        const uint32_t o = ok ? off : 0x80000000u;
+#pragma unroll
+        for (int v = 0; v < LR / 16; ++v) {
+            const uint4 w = reinterpret_cast<const uint4*>(src)[v];
+            __builtin_amdgcn_raw_buffer_store_b128(pyr_u32x4{w.x, w.y, w.z, w.w}, rs, (int)(o + 16 * v), 0, 0);
+        }
+    }
+};
 
 constexpr int kPkDeltaLo = -254, kPkDeltaHi = 256;   // table range of b - b'
 // Blocks of cost loads in flight per wave.  A block is LR / 2 steps; its
@@ -431,7 +468,12 @@ __device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, in
 
     pk16 L = pk_of(0u), m = pk_of(0u);
     int last_b = 0;
+    // A11: vertical directions seed b' from row 0 whichever end they start at
+    const int seed_b = horiz ? 0 : a.init[line];
     const int nblk = (n + K - 1) / K;
+    const BlockStores<LR, K> stores(horiz, fwd, W, n, group * SPW);
+    const uint32_t woff = BlockStores<LR, K>::offset(horiz, fwd, W, lsub, lstep);
+    const bool wline = lline < nlines;
     auto process = [&](int blk) __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
         // the bases of the block's K steps of this lane's line and this
@@ -456,8 +498,7 @@ __device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, in
             if (s == 0 && blk == 0) {
                 L = c;
                 m = pk_group_min<LR>(L);
-                // A11: vertical directions seed b' from row 0 whichever end they start at
-                last_b = horiz ? b : a.init[line];
+                last_b = horiz ? b : seed_b;
                 o = L;
             } else {
                 const bool valid = b != kInvalidShort;
@@ -508,28 +549,27 @@ __device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, in
                 (uint16_t)__builtin_amdgcn_perm(0u, pk_bits(o), 0x0c0c0200u);
         }
         __builtin_amdgcn_wave_barrier();
-        {   // this block's results, one window per lane
-            const int step = blk * K + lstep;
-            if (lline < nlines && step < n) {
-                uint4* dst = reinterpret_cast<uint4*>(out + pix_of(lline, step) * LR);
-                const uint4* src = reinterpret_cast<const uint4*>(S.out + lane * LR);
-#pragma unroll
-                for (int qv = 0; qv < NV; ++qv) dst[qv] = src[qv];
-            }
-        }
+        // this block's results, one window per lane
+        stores.store(out, blk, woff, wline && lstep < n - blk * K, S.out + lane * LR);
     };
+    // the prologue issues the ring's slots in the loop's order: the wait
+    // counts at the loop head merge both paths
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < kPkAhead; ++u) fetch(u, u);
+    for (int u = 0; u < kPkAhead; ++u) {
+        fetch(u, u);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     for (int blk0 = 0; blk0 < nblk; blk0 += kPkAhead) {
 #pragma unroll
         for (int u = 0; u < kPkAhead; ++u) {
+            // blocks past the last one (at most kPkAhead - 1) walk the
+            // line's last pixel again; their stores are dropped
             const int blk = blk0 + u;
-            if (blk < nblk) {
-                __builtin_amdgcn_wave_barrier();
-                stage(u);
-                if (blk + kPkAhead < nblk) fetch(u, blk + kPkAhead);
-                process(blk);
-            }
+            __builtin_amdgcn_wave_barrier();
+            stage(u);
+            fetch(u, blk + kPkAhead);
+            process(blk);
         }
     }
 }
@@ -665,7 +705,43 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
 
     pk16 LA = pk_of(0u), LB = pk_of(0u), m = pk_of(0u);
     int last_b = 0;
+    // A11: vertical directions seed b' from row 0 whichever end they start at
+    const int seed_b = horiz ? 0 : a.init[line];
     const int nblk = (n + K - 1) / K;
+    const BlockStores<LR, K> stores(horiz, fwd, W, n, group * SPW);
+    uint32_t woff[2];
+    bool wline[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int idx = lane + 64 * h;
+        woff[h] = BlockStores<LR, K>::offset(horiz, fwd, W, idx / K, idx % K);
+        wline[h] = group * SPW + idx / K < nlines;
+    }
+    // neighbour pairs at 4k + q - 1, 4k + q + 1, 4k + q + 3 for a base step q
+    // in {-1, 0, 1}: bytes 2 (q + 1) .. 2 (q + 1) + 3 of {oa, prv}, {ob, oa},
+    // {nxt, ob}, with `sel` the selector of q
+    auto dpp_terms = [&](uint32_t oa, uint32_t ob, uint32_t sel, int q, uint32_t& x0, uint32_t& x1,
+                         uint32_t& x2, pk16& pen2) __attribute__((always_inline)) {
+        uint32_t nxt = dppu<0x101>(oa);       // lane k + 1's A
+        uint32_t prv = dppu<0x111>(ob);       // lane k - 1's B
+        nxt = last_k ? kPadPair : nxt;
+        prv = first_k ? kPadPair : prv;
+        x0 = __builtin_amdgcn_perm(oa, prv, sel);
+        x1 = __builtin_amdgcn_perm(ob, oa, sel);
+        x2 = __builtin_amdgcn_perm(nxt, ob, sel);
+        const uint32_t mneg = 0u - (uint32_t)(q < 0);
+        pen2 = pk_of((pk_bits(pen2_m1) & mneg) | (pk_bits(pen2_01) & ~mneg));
+    };
+    // this step's L of the lane's two pairs from the previous L, m and the
+    // neighbour terms
+    auto relax = [&](pk16 cA, pk16 cB, uint32_t x0, uint32_t x1, uint32_t x2, pk16 pen2, pk16& lA,
+                     pk16& lB) __attribute__((always_inline)) {
+        const pk16 t = __builtin_elementwise_min(pk_of(x1), m + pen2);
+        const pk16 vA = __builtin_elementwise_min(__builtin_elementwise_min(LA, pk_of(x0)), t);
+        const pk16 vB = __builtin_elementwise_min(__builtin_elementwise_min(LB, pk_of(x2)), t);
+        lA = cA + (vA - m);
+        lB = cB + (vB - m);
+    };
     auto process = [&](int blk) __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
         uint32_t bw[K / 2], ca[K], cb[K];
@@ -690,8 +766,7 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
                 LA = cA;
                 LB = cB;
                 m = group_min4(__builtin_elementwise_min(LA, LB));
-                // A11: vertical directions seed b' from row 0 whichever end they start at
-                last_b = horiz ? b : a.init[line];
+                last_b = horiz ? b : seed_b;
                 ow = __builtin_amdgcn_perm(pk_bits(LB), pk_bits(LA), 0x06040200u);
             } else {
                 const bool valid = b != kInvalidShort;
@@ -700,18 +775,7 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
                 uint32_t x0, x1, x2;                     // pairs at 4k + q - 1, + 1, + 3
                 pk16 pen2;
                 if (__builtin_expect(__builtin_amdgcn_ballot_w64(valid && (q < -1 || q > 1)) == 0, 1)) {
-                    uint32_t nxt = dppu<0x101>(oa);       // lane k + 1's A
-                    uint32_t prv = dppu<0x111>(ob);       // lane k - 1's B
-                    nxt = last_k ? kPadPair : nxt;
-                    prv = first_k ? kPadPair : prv;
-                    // pairs at 4k + q - 1, 4k + q + 1, 4k + q + 3: bytes
-                    // 2 (q + 1) .. 2 (q + 1) + 3 of {oa, prv}, {ob, oa}, {nxt, ob}
-                    const uint32_t sel = shift_sel(q, 0x0202u, k10001, 0x03020100u);
-                    x0 = __builtin_amdgcn_perm(oa, prv, sel);
-                    x1 = __builtin_amdgcn_perm(ob, oa, sel);
-                    x2 = __builtin_amdgcn_perm(nxt, ob, sel);
-                    const uint32_t mneg = 0u - (uint32_t)(q < 0);
-                    pen2 = pk_of((pk_bits(pen2_m1) & mneg) | (pk_bits(pen2_01) & ~mneg));
+                    dpp_terms(oa, ob, shift_sel(q, 0x0202u, k10001, 0x03020100u), q, x0, x1, x2, pen2);
                 } else {
                     put(oa, ob);   // the window row holds L only when a step needs it
                     const int qt = min(max(q, qlo), qhi);
@@ -724,10 +788,8 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
                     asm volatile("" : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(pb));
                     pen2 = pk_of(pb);
                 }
-                const pk16 t = __builtin_elementwise_min(pk_of(x1), m + pen2);
-                const pk16 vA = __builtin_elementwise_min(__builtin_elementwise_min(LA, pk_of(x0)), t);
-                const pk16 vB = __builtin_elementwise_min(__builtin_elementwise_min(LB, pk_of(x2)), t);
-                const pk16 lA = cA + (vA - m), lB = cB + (vB - m);
+                pk16 lA, lB;
+                relax(cA, cB, x0, x1, x2, pen2, lA, lB);
                 const pk16 mn = group_min4(__builtin_elementwise_min(lA, lB));
                 const uint32_t vm = 0u - (uint32_t)valid;
                 ow = __builtin_amdgcn_perm(pk_bits(lB), pk_bits(lA), 0x06040200u) & vm;
@@ -742,27 +804,27 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
 #pragma unroll
         for (int h = 0; h < 2; ++h) {   // this block's results, one window per pixel
             const int idx = lane + 64 * h;
-            const int ll = group * SPW + idx / K, step = blk * K + idx % K;
-            if (ll < nlines && step < n) {
-                uint4* dst = reinterpret_cast<uint4*>(out + pix_of(ll, step) * LR);
-                const uint4* src = reinterpret_cast<const uint4*>(S.out + idx * LR);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) dst[v] = src[v];
-            }
+            stores.store(out, blk, woff[h], wline[h] && idx % K < n - blk * K, S.out + idx * LR);
         }
     };
+    // the prologue issues the ring's slots in the loop's order: the wait
+    // counts at the loop head merge both paths
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < kPkAhead; ++u) fetch(u, u);
+    for (int u = 0; u < kPkAhead; ++u) {
+        fetch(u, u);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     for (int blk0 = 0; blk0 < nblk; blk0 += kPkAhead) {
 #pragma unroll
         for (int u = 0; u < kPkAhead; ++u) {
+            // blocks past the last one (at most kPkAhead - 1) walk the
+            // line's last pixel again; their stores are dropped
             const int blk = blk0 + u;
-            if (blk < nblk) {
-                __builtin_amdgcn_wave_barrier();
-                stage(u);
-                if (blk + kPkAhead < nblk) fetch(u, blk + kPkAhead);
-                process(blk);
-            }
+            __builtin_amdgcn_wave_barrier();
+            stage(u);
+            fetch(u, blk + kPkAhead);
+            process(blk);
         }
     }
 }
