@@ -82,6 +82,25 @@ def test_window_aggregation_penalty_forms(cuda, window_form, lr, p1, p2):
 
 
 @pytest.mark.parametrize("lr", [16, 32])
+@pytest.mark.parametrize("h,w", [(1, 1), (1, 7), (3, 1), (2, 17), (5, 9), (17, 3), (9, 33), (1, 40), (40, 1)])
+def test_window_aggregation_tiny_shapes(cuda, window_form, lr, h, w):
+    """Scanlines shorter than one block of steps and fewer scanlines than a
+    wave holds: the blocks past a line's end and the pixels past the last
+    scanline are computed but their stores are dropped; every pixel of the
+    four direction volumes must match the oracle."""
+    print(f"[window aggregation tiny] lr={lr} {h}x{w} form={window_form}", flush=True)
+    rng = np.random.default_rng(h * 100 + w + lr)
+    init = (rng.integers(0, 3, size=(h, w)).cumsum(axis=1) - 1).astype(np.int16)   # steps of -1..1
+    init[rng.random((h, w)) < 0.15] = oracle.INVALID_SHORT
+    cost = rng.integers(0, 64, size=(h, w, lr), dtype=np.uint8)
+    paths = stages.aggregate_init(torch.from_numpy(cost).to(cuda), torch.from_numpy(init).to(cuda),
+                                  10, 150)
+    for r in range(4):
+        np.testing.assert_array_equal(paths[r].cpu().numpy(),
+                                      oracle.aggregate_init(cost, init, r, 10, 150), err_msg=f"dir {r}")
+
+
+@pytest.mark.parametrize("lr", [16, 32])
 @pytest.mark.parametrize("p1,p2", [(127, 254), (127, 127), (0, 0), (1, 254), (128, 254), (127, 255)])
 def test_window_aggregation_packed_limits(cuda, window_form, lr, p1, p2):
     """The packed-u16 window aggregation (P1 <= 127, P2 <= 254) at the edges
