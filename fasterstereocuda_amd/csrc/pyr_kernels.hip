@@ -316,8 +316,9 @@ struct BlockStores {
         return (uint32_t)((horiz ? ln * W + jj : jj * W + ln) * LR);
     }
     __device__ void store(uint8_t* out, int blk, uint32_t off, bool ok, const uint8_t* src) const {
-        // the base of a block past the line's end may lie before the volume;
-        // none of its stores is issued in range
+        // in the reverse directions the base of a line's last, partial block
+        // (and of the blocks past it) lies before the volume: only pixels
+        // inside the line are in range, and their offsets land inside it
         const uintptr_t base = (uintptr_t)out + (uintptr_t)((base0 + blk * step) * LR);
         const __amdgpu_buffer_rsrc_t rs =
             __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, 0x7fffffff, 0x00020000);
