@@ -617,7 +617,14 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
     constexpr int NV = LR / 16;
     constexpr uint32_t kPadPair = 127u * 0x10001u;
     static_assert(SPW * K == 128, "a block is two pixels per lane");
-    const int sub = lane / LPL, k = lane % LPL;      // recurrence: line, disparity quad
+    // The lanes of a scanline sit STRIDE = 16 / LPL apart in one DPP row, so a
+    // neighbouring quad is a row shift by STRIDE.  The line's first and last
+    // lanes then read from outside the row, which leaves a move's old value in
+    // place: the 127 pads stay in those lanes of two persistent registers,
+    // with no select per step.
+    constexpr int STRIDE = 16 / LPL;
+    const int sub = (lane / 16) * STRIDE + lane % STRIDE;   // recurrence: line
+    const int k = (lane % 16) / STRIDE;                     // disparity quad
     const bool horiz = (dir == 1 || dir == 3);
     const bool fwd = (dir == 0 || dir == 1);
     const int W = a.w, H = a.h;
@@ -676,7 +683,7 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
 
     uint32_t* wrow = &S.win[sub][0];
     const uint32_t* wlane = wrow + (PAD - 1 + 4 * k);     // + q: the first neighbour pair
-    const bool first_k = k == 0, last_k = k == LPL - 1;
+    const bool last_k = k == LPL - 1;
     uint8_t* obase = S.out + sub * K * LR + 4 * k;
     const pk16 p1v = pk_of((uint32_t)a.p1 * 0x10001u);
     const pk16 m127 = pk_of(kPadPair);
@@ -688,7 +695,7 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
     // the window row from this lane's pairs of L + P1 (A, B): C[PAD + 4k ..
     // PAD + 4k + 3] in one 16-byte store, and C[PAD - 1] from lane 0
     auto put = [&](uint32_t oa, uint32_t ob) __attribute__((always_inline)) {
-        uint32_t nxt = dppu<0x101>(oa);                   // row_shl:1, lane k + 1's A
+        uint32_t nxt = dppu<0x100 + STRIDE>(oa);          // row_shl: lane k + 1's A
         nxt = last_k ? kPadPair : nxt;
         const uint4 v = make_uint4(oa, __builtin_amdgcn_alignbit(ob, oa, 16), ob,
                                    __builtin_amdgcn_alignbit(nxt, ob, 16));
@@ -698,9 +705,9 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
     auto group_min4 = [&](pk16 v) __attribute__((always_inline)) {
         v = __builtin_elementwise_min(v, v.yx);
         uint32_t m = pk_bits(v);
-        m = min(m, dppu<0xB1>(m));                        // quad_perm [1, 0, 3, 2]
-        m = min(m, dppu<0x4E>(m));                        // quad_perm [2, 3, 0, 1]
-        if constexpr (LPL == 8) m = min(m, dppu<0x141>(m));   // row_half_mirror
+        m = min(m, dppu<0x120 + STRIDE>(m));              // row_ror by STRIDE, 2 STRIDE, ...
+        if constexpr (2 * STRIDE < 16) m = min(m, dppu<0x120 + 2 * STRIDE>(m));
+        if constexpr (4 * STRIDE < 16) m = min(m, dppu<0x120 + 4 * STRIDE>(m));
         return pk_of(m);
     };
 
@@ -718,18 +725,19 @@ __device__ __forceinline__ void agg_init_pk2_lines(const PyrLevel& a, int dir, i
         woff[h] = BlockStores<LR, K>::offset(horiz, fwd, W, idx / K, idx % K);
         wline[h] = group * SPW + idx / K < nlines;
     }
+    uint32_t nxt_r = kPadPair, prv_r = kPadPair;   // pads kept in the last / first lane of a line
     // neighbour pairs at 4k + q - 1, 4k + q + 1, 4k + q + 3 for a base step q
     // in {-1, 0, 1}: bytes 2 (q + 1) .. 2 (q + 1) + 3 of {oa, prv}, {ob, oa},
     // {nxt, ob}, with `sel` the selector of q
     auto dpp_terms = [&](uint32_t oa, uint32_t ob, uint32_t sel, int q, uint32_t& x0, uint32_t& x1,
                          uint32_t& x2, pk16& pen2) __attribute__((always_inline)) {
-        uint32_t nxt = dppu<0x101>(oa);       // lane k + 1's A
-        uint32_t prv = dppu<0x111>(ob);       // lane k - 1's B
-        nxt = last_k ? kPadPair : nxt;
-        prv = first_k ? kPadPair : prv;
-        x0 = __builtin_amdgcn_perm(oa, prv, sel);
+        // lane k + 1's A and lane k - 1's B (row_shl / row_shr by STRIDE,
+        // out-of-row sources not written)
+        nxt_r = (uint32_t)__builtin_amdgcn_update_dpp((int)nxt_r, (int)oa, 0x100 + STRIDE, 0xF, 0xF, false);
+        prv_r = (uint32_t)__builtin_amdgcn_update_dpp((int)prv_r, (int)ob, 0x110 + STRIDE, 0xF, 0xF, false);
+        x0 = __builtin_amdgcn_perm(oa, prv_r, sel);
         x1 = __builtin_amdgcn_perm(ob, oa, sel);
-        x2 = __builtin_amdgcn_perm(nxt, ob, sel);
+        x2 = __builtin_amdgcn_perm(nxt_r, ob, sel);
         const uint32_t mneg = 0u - (uint32_t)(q < 0);
         pen2 = pk_of((pk_bits(pen2_m1) & mneg) | (pk_bits(pen2_01) & ~mneg));
     };
