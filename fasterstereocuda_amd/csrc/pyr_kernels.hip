@@ -369,16 +369,17 @@ __device__ __forceinline__ uint32_t shift_sel(int q, uint32_t k0202, uint32_t k1
     return sel;
 }
 
-// minimum over each aligned group of LR / 2 lanes of both halves; the result
-// holds it in both halves
+// minimum over the LR / 2 lanes of a scanline (STRIDE = 32 / LR apart in a
+// DPP row) of both halves; the result holds it in both halves
 template <int LR>
 __device__ __forceinline__ pk16 pk_group_min(pk16 v) {
+    constexpr int STRIDE = 32 / LR;
     v = __builtin_elementwise_min(v, v.yx);
     uint32_t m = pk_bits(v);                 // equal halves: u32 order = u16 order
-    m = min(m, dppu<0xB1>(m));               // quad_perm [1, 0, 3, 2]
-    m = min(m, dppu<0x4E>(m));               // quad_perm [2, 3, 0, 1]
-    m = min(m, dppu<0x141>(m));              // row_half_mirror
-    if constexpr (LR == 32) m = min(m, dppu<0x140>(m));   // row_mirror
+    m = min(m, dppu<0x120 + STRIDE>(m));     // row_ror by STRIDE, 2 STRIDE, ...
+    m = min(m, dppu<0x120 + 2 * STRIDE>(m));
+    m = min(m, dppu<0x120 + 4 * STRIDE>(m));
+    if constexpr (8 * STRIDE < 16) m = min(m, dppu<0x120 + 8 * STRIDE>(m));
     return pk_of(m);
 }
 
@@ -389,7 +390,11 @@ __device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, in
     constexpr int PAD = AggInitPkLds<LR>::PAD, ROW = AggInitPkLds<LR>::ROW;
     constexpr int NV = LR / 16;
     constexpr uint32_t kPadPair = 127u * 0x10001u;
-    const int sub = lane / LPL, k = lane % LPL;      // recurrence: line, disparity pair
+    // the lanes of a scanline sit STRIDE = 16 / LPL apart in one DPP row (see
+    // agg_init_pk2_lines): the edge pads stay in unwritten lanes
+    constexpr int STRIDE = 16 / LPL;
+    const int sub = (lane / 16) * STRIDE + lane % STRIDE;   // recurrence: line
+    const int k = (lane % 16) / STRIDE;                     // disparity pair
     const int lsub = lane / K, lstep = lane % K;     // loads / stores: line, step
     const bool horiz = (dir == 1 || dir == 3);
     const bool fwd = (dir == 0 || dir == 1);
@@ -452,7 +457,6 @@ __device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, in
     // rematerialised each step
     int qlo = kPkDeltaLo, qhi = kPkDeltaHi, wlo = -(LR + 1);
     asm volatile("" : "+v"(qlo), "+v"(qhi), "+v"(wlo));
-    const bool first_k = k == 0;
     // P2 terms of the offsets -1, 0 and 1 (divisors 2, 1 and 1)
     const pk16 pen2_m1 = pk_of(S.pen2[-1 - kPkDeltaLo]);
     const pk16 pen2_01 = pk_of(S.pen2[0 - kPkDeltaLo]);
@@ -460,7 +464,7 @@ __device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, in
     // pair), C[PAD + 2k + 1] (its high half and the next lane's low half or a
     // pad) and, from lane 0, C[PAD - 1] (a pad and its low half)
     auto put = [&](uint32_t own) __attribute__((always_inline)) {
-        uint32_t nxt = dppu<0x101>(own);                  // row_shl:1, lane k + 1
+        uint32_t nxt = dppu<0x100 + STRIDE>(own);         // row_shl: lane k + 1
         nxt = last_k ? kPadPair : nxt;
         const uint32_t mid = __builtin_amdgcn_alignbit(nxt, own, 16);
         *reinterpret_cast<uint2*>(wrow + PAD + 2 * k) = make_uint2(own, mid);
@@ -468,6 +472,7 @@ __device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, in
     };
 
     pk16 L = pk_of(0u), m = pk_of(0u);
+    uint32_t nxt_r = kPadPair, prv_r = kPadPair;   // pads kept in the last / first lane of a line
     int last_b = 0;
     // A11: vertical directions seed b' from row 0 whichever end they start at
     const int seed_b = horiz ? 0 : a.init[line];
@@ -509,10 +514,13 @@ __device__ __forceinline__ void agg_init_pk_lines(const PyrLevel& a, int dir, in
                 if (__builtin_expect(__builtin_amdgcn_ballot_w64(valid && (q < -1 || q > 1)) == 0, 1)) {
                     // every line shifts by at most one disparity: the terms are
                     // this lane's pair and its neighbours' (DPP), no LDS
-                    uint32_t nxt = dppu<0x101>(own);          // row_shl:1, lane k + 1
-                    uint32_t prv = dppu<0x111>(own);          // row_shr:1, lane k - 1
-                    nxt = last_k ? kPadPair : nxt;
-                    prv = first_k ? kPadPair : prv;
+                    // lane k + 1 and lane k - 1 (row_shl / row_shr by STRIDE,
+                    // out-of-row sources not written: the pads stay)
+                    nxt_r = (uint32_t)__builtin_amdgcn_update_dpp((int)nxt_r, (int)own, 0x100 + STRIDE, 0xF, 0xF,
+                                                                  false);
+                    prv_r = (uint32_t)__builtin_amdgcn_update_dpp((int)prv_r, (int)own, 0x110 + STRIDE, 0xF, 0xF,
+                                                                  false);
+                    const uint32_t nxt = nxt_r, prv = prv_r;
                     // the pair starting at disparity 2k + q - 1 (2k + q + 1) is
                     // bytes 2 (q + 1) .. 2 (q + 1) + 3 of {own, prv} ({nxt, own})
                     const uint32_t sel = shift_sel(q, 0x0202u, k10001, 0x03020100u);
